@@ -1,0 +1,879 @@
+// Host side of the C ABI declared in include/culbm.h: context, buffers,
+// policy dispatch and the temporal-blocking wavefront schedule.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/culbm.h"
+#include "lbm_kernels.hip"
+
+using namespace culbm;
+
+struct culbm_ctx {
+  culbm_config cfg{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  int64_t n = 0;           // cells
+  int32_t px = 0, py = 0, pz = 0;
+  int64_t dstride = 0;     // population elements per direction
+  int32_t bpp = 4;         // bytes per population
+  bool inplace = false;
+  void* buf[2] = {nullptr, nullptr};
+  int cur = 0;             // pull: index of the buffer holding the current state
+  int parity = 1;          // in-place: parity of the next step (1 = O, 0 = E)
+  uint32_t* flag = nullptr;
+  uint8_t* sflag = nullptr;
+  float* rho = nullptr;
+  float* vx = nullptr;
+  float* vy = nullptr;
+  float* vz = nullptr;
+  int64_t step = 0;
+  float scale = 1.0f, inv_scale = 1.0f;
+  int32_t wave_depth = 0;
+  int32_t coop_blocks = 0;
+  int64_t device_bytes = 0;
+  // per-launch timing (culbm_profile_steps): an event pair around every launch
+  bool prof = false;
+  size_t prof_used = 0;
+  std::vector<hipEvent_t> prof_ev;
+};
+
+namespace {
+
+thread_local std::string g_create_err = "no error";
+
+int fail(culbm_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx)
+    ctx->err = buf;
+  else
+    g_create_err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), CULBM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+bool is_pow2_scale(float s) {
+  if (!(s > 0.0f) || !std::isfinite(s)) return false;
+  int e;
+  const float m = std::frexp(s, &e);
+  return m == 0.5f;
+}
+
+bool blocked(const culbm_config& c) { return c.opt_policy != CULBM_OPT_NONE; }
+
+// Default scale of the 16-bit formats.  Shifted populations stay well below
+// 1 in magnitude, so FP16 takes 2^12 to move them away from its subnormal
+// range; BF16 has FP32's exponent range and needs none.
+float default_scale(int32_t storage) { return storage == CULBM_STORAGE_FP16 ? 4096.0f : 1.0f; }
+
+}  // namespace
+
+extern "C" {
+
+void culbm_config_default(culbm_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->nx = c->ny = c->nz = 256;
+  c->inv_tau = 0.5f;
+  c->stream_policy = CULBM_STREAM_PULL;
+  c->opt_policy = CULBM_OPT_NONE;
+  c->inner_loop = 5;
+  c->block_dim[0] = 32;
+  c->block_dim[1] = 16;
+  c->block_dim[2] = 2;
+  c->grid_dim[0] = 2;
+  c->grid_dim[1] = 3;
+  c->grid_dim[2] = 19;
+  c->use_ring = 0;
+  c->device = 0;
+  c->storage = CULBM_STORAGE_FP32;
+  c->storage_scale = 0.0f;
+  c->wave_depth = 0;
+}
+
+int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
+  auto bad = [&](int code, const char* fmt, ...) {
+    if (msg && msg_len) {
+      va_list ap;
+      va_start(ap, fmt);
+      vsnprintf(msg, msg_len, fmt, ap);
+      va_end(ap);
+    }
+    return code;
+  };
+  if (!c) return bad(CULBM_ERR_INVALID, "config is NULL");
+  if (c->nx < 1 || c->ny < 1 || c->nz < 1)
+    return bad(CULBM_ERR_INVALID, "domain dimensions must be >= 1 (got %d x %d x %d)", c->nx, c->ny,
+               c->nz);
+  const int64_t n = int64_t(c->nx) * c->ny * c->nz;
+  const int64_t padded = round_up(int64_t(c->nx) + 1, 64) * (int64_t(c->ny) + 1) * (int64_t(c->nz) + 1);
+  if (n > (int64_t(1) << 31) - 1 || padded > (int64_t(1) << 32) - 1)
+    return bad(CULBM_ERR_INVALID, "domain of %lld cells exceeds the 32-bit cell index", (long long)n);
+  if (!std::isfinite(c->inv_tau)) return bad(CULBM_ERR_INVALID, "invTau must be finite");
+  if (c->stream_policy != CULBM_STREAM_PULL && c->stream_policy != CULBM_STREAM_INPLACE)
+    return bad(CULBM_ERR_INVALID, "streamPolicy must be 0 (pull) or 1 (in-place), got %d",
+               c->stream_policy);
+  if (c->opt_policy < CULBM_OPT_NONE || c->opt_policy > CULBM_OPT_DYNAMIC)
+    return bad(CULBM_ERR_INVALID, "optPolicy must be in 0..3, got %d", c->opt_policy);
+  if (c->opt_policy == CULBM_OPT_MIXED)
+    return bad(CULBM_ERR_UNSUPPORTED,
+               "optPolicy 2 (HALO_BLOCKING_L1L2) is not provided by this engine");
+  // same rule as the reference (simulator_expt_platform.cu:365-368)
+  if (c->use_ring &&
+      (c->stream_policy != CULBM_STREAM_INPLACE || c->opt_policy != CULBM_OPT_DYNAMIC))
+    return bad(CULBM_ERR_INVALID,
+               "--useRingDDFBuf is unimplemented unless --streamPolicy 1 --optPolicy 3.");
+  if (blocked(*c) && c->inner_loop < 1)
+    return bad(CULBM_ERR_INVALID, "innerLoop must be >= 1, got %d", c->inner_loop);
+  if (c->storage < CULBM_STORAGE_FP32 || c->storage > CULBM_STORAGE_BF16)
+    return bad(CULBM_ERR_INVALID, "storage must be 0 (fp32), 1 (fp16) or 2 (bf16), got %d", c->storage);
+  if (c->storage == CULBM_STORAGE_FP32 && c->storage_scale != 0.0f && c->storage_scale != 1.0f)
+    return bad(CULBM_ERR_INVALID, "FP32 storage takes no scale");
+  if (c->storage != CULBM_STORAGE_FP32 && c->storage_scale != 0.0f && !is_pow2_scale(c->storage_scale))
+    return bad(CULBM_ERR_INVALID, "storage scale must be a positive power of two, got %g",
+               double(c->storage_scale));
+  if (c->wave_depth < 0) return bad(CULBM_ERR_INVALID, "wave depth must be >= 0");
+  if (c->device < 0) return bad(CULBM_ERR_INVALID, "devId must be >= 0");
+  if (msg && msg_len) msg[0] = 0;
+  return CULBM_OK;
+}
+
+const char* culbm_error_string(int code) {
+  switch (code) {
+    case CULBM_OK: return "CULBM_OK";
+    case CULBM_ERR_INVALID: return "CULBM_ERR_INVALID";
+    case CULBM_ERR_HIP: return "CULBM_ERR_HIP";
+    case CULBM_ERR_STATE: return "CULBM_ERR_STATE";
+    case CULBM_ERR_UNSUPPORTED: return "CULBM_ERR_UNSUPPORTED";
+    default: return "CULBM_ERR_UNKNOWN";
+  }
+}
+
+const char* culbm_last_error(const culbm_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+}  // extern "C"
+
+namespace {
+
+StepArgs base_args(const culbm_ctx* ctx) {
+  StepArgs a{};
+  a.nx = ctx->cfg.nx;
+  a.ny = ctx->cfg.ny;
+  a.nz = ctx->cfg.nz;
+  a.px = ctx->px;
+  a.py = ctx->py;
+  a.z0 = 0;
+  a.z1 = ctx->cfg.nz;
+  a.ncells = uint32_t(ctx->n);
+  a.dstride = ctx->dstride;
+  a.divx = FastDiv::make(uint32_t(ctx->cfg.nx));
+  a.divy = FastDiv::make(uint32_t(ctx->cfg.ny));
+  a.sflag = ctx->sflag;
+  a.rho = ctx->rho;
+  a.vx = ctx->vx;
+  a.vy = ctx->vy;
+  a.vz = ctx->vz;
+  a.inv_tau = ctx->cfg.inv_tau;
+  a.omt = 1.0f - ctx->cfg.inv_tau;
+  a.scale = ctx->scale;
+  a.inv_scale = ctx->inv_scale;
+  return a;
+}
+
+void set_range(StepArgs& a, int32_t z0, int32_t z1) {
+  a.z0 = z0;
+  a.z1 = z1;
+  a.ncells = uint32_t(z1 - z0) * uint32_t(a.nx) * uint32_t(a.ny);
+  a.nblk = (a.ncells + kBlock - 1) / kBlock;
+}
+
+// Event pair around one launch when profiling is on.
+int prof_begin(culbm_ctx* ctx) {
+  if (!ctx->prof) return CULBM_OK;
+  while (ctx->prof_ev.size() < 2 * (ctx->prof_used + 1)) {
+    hipEvent_t e;
+    HIPCHK(ctx, hipEventCreate(&e));
+    ctx->prof_ev.push_back(e);
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->prof_ev[2 * ctx->prof_used], ctx->stream));
+  return CULBM_OK;
+}
+
+int prof_end(culbm_ctx* ctx) {
+  if (!ctx->prof) return CULBM_OK;
+  HIPCHK(ctx, hipEventRecord(ctx->prof_ev[2 * ctx->prof_used + 1], ctx->stream));
+  ++ctx->prof_used;
+  return CULBM_OK;
+}
+
+template <class S>
+int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) {
+  set_range(a, z0, z1);
+  if (a.ncells == 0) return CULBM_OK;
+  if (int rc = prof_begin(ctx)) return rc;
+  if (ctx->inplace) {
+    a.src = ctx->buf[0];
+    a.dst = ctx->buf[0];
+    if (parity)
+      hipLaunchKernelGGL((k_twist<S, 1>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((k_twist<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_pull<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return prof_end(ctx);
+}
+
+// One temporal-blocking pass of T steps as a skewed z-wavefront.  Chunk k at
+// step t (1-based) covers planes [kD - (t-1), (k+1)D - (t-1)): every plane it
+// reads at level t-1 was produced either by an earlier chunk or by the same
+// chunk one launch earlier, and no (plane, step) is computed twice.  A chunk's
+// working set (D + T planes) is sized for the Infinity Cache.
+template <class S>
+int wave_pass(culbm_ctx* ctx, int32_t T) {
+  StepArgs a = base_args(ctx);
+  const int32_t nz = ctx->cfg.nz, D = ctx->wave_depth;
+  const int32_t K = (nz + T - 1 + D - 1) / D;
+  for (int32_t k = 0; k < K; ++k) {
+    for (int32_t t = 1; t <= T; ++t) {
+      const int32_t lo = std::max(0, k * D - (t - 1));
+      const int32_t hi = std::min(nz, (k + 1) * D - (t - 1));
+      if (lo >= hi) continue;
+      if (!ctx->inplace) {
+        a.src = ctx->buf[(ctx->cur + t - 1) & 1];
+        a.dst = ctx->buf[(ctx->cur + t) & 1];
+      }
+      const int rc = launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1);
+      if (rc) return rc;
+    }
+  }
+  ctx->cur = (ctx->cur + T) & 1;
+  ctx->parity = (ctx->parity + T) & 1;
+  return CULBM_OK;
+}
+
+template <class S>
+int coop_pass(culbm_ctx* ctx, int32_t T) {
+  WaveArgs w{};
+  w.a = base_args(ctx);
+  w.depth = ctx->wave_depth;
+  w.nsteps = T;
+  w.parity0 = ctx->parity;
+  if (ctx->inplace) {
+    w.a.src = ctx->buf[0];
+    w.a.dst = ctx->buf[0];
+  } else {
+    w.buf0 = ctx->buf[ctx->cur];
+    w.buf1 = ctx->buf[ctx->cur ^ 1];
+  }
+  void* args[] = {&w};
+  if (int rc = prof_begin(ctx)) return rc;
+  const void* fn = ctx->inplace ? reinterpret_cast<const void*>(&k_wave_coop<S, true>)
+                                : reinterpret_cast<const void*>(&k_wave_coop<S, false>);
+  HIPCHK(ctx, hipLaunchCooperativeKernel(fn, dim3(ctx->coop_blocks), dim3(kBlock), args, 0,
+                                         ctx->stream));
+  if (int rc = prof_end(ctx)) return rc;
+  ctx->cur = (ctx->cur + T) & 1;
+  ctx->parity = (ctx->parity + T) & 1;
+  return CULBM_OK;
+}
+
+template <class S>
+int do_steps(culbm_ctx* ctx, int32_t nsteps) {
+  const culbm_config& c = ctx->cfg;
+  if (!blocked(c)) {
+    StepArgs a = base_args(ctx);
+    for (int32_t s = 0; s < nsteps; ++s) {
+      if (!ctx->inplace) {
+        a.src = ctx->buf[ctx->cur];
+        a.dst = ctx->buf[ctx->cur ^ 1];
+      }
+      const int rc = launch_step<S>(ctx, a, 0, c.nz, ctx->parity);
+      if (rc) return rc;
+      ctx->cur ^= 1;
+      ctx->parity ^= 1;
+    }
+  } else {
+    int32_t left = nsteps;
+    while (left > 0) {
+      const int32_t T = std::min(left, c.inner_loop);
+      const int rc = c.opt_policy == CULBM_OPT_STATIC ? coop_pass<S>(ctx, T) : wave_pass<S>(ctx, T);
+      if (rc) return rc;
+      left -= T;
+    }
+  }
+  ctx->step += nsteps;
+  return CULBM_OK;
+}
+
+int steps_dispatch(culbm_ctx* ctx, int32_t nsteps) {
+  switch (ctx->cfg.storage) {
+    case CULBM_STORAGE_FP16: return do_steps<StoreF16>(ctx, nsteps);
+    case CULBM_STORAGE_BF16: return do_steps<StoreBF16>(ctx, nsteps);
+    default: return do_steps<StoreF32>(ctx, nsteps);
+  }
+}
+
+StepArgs full_args(const culbm_ctx* ctx) {
+  StepArgs a = base_args(ctx);
+  set_range(a, 0, ctx->cfg.nz);
+  return a;
+}
+
+// plain FP32 device array -> populations in the context's layout
+template <class S>
+int upload_plain(culbm_ctx* ctx, const float* dplain) {
+  StepArgs a = full_args(ctx);
+  if (ctx->inplace) {
+    // written with the E-step store rules, so the next step is an O step
+    hipLaunchKernelGGL((k_plain_to_twist<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                       dplain, ctx->buf[0]);
+    ctx->parity = 1;
+  } else {
+    for (int b = 0; b < 2; ++b)
+      hipLaunchKernelGGL((k_plain_to_pull<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                         dplain, ctx->buf[b]);
+    ctx->cur = 0;
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return CULBM_OK;
+}
+
+template <class S>
+int download_plain(culbm_ctx* ctx, float* dplain) {
+  StepArgs a = full_args(ctx);
+  if (ctx->inplace) {
+    if (ctx->parity == 1)  // last stores used the E rules
+      hipLaunchKernelGGL((k_twist_to_plain<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                         ctx->buf[0], dplain);
+    else
+      hipLaunchKernelGGL((k_twist_to_plain<S, 1>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                         ctx->buf[0], dplain);
+  } else {
+    hipLaunchKernelGGL((k_pull_to_plain<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                       ctx->buf[ctx->cur], dplain);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return CULBM_OK;
+}
+
+template <class S>
+int dump_impl(culbm_ctx* ctx) {
+  DumpArgs d{};
+  d.a = full_args(ctx);
+  d.flag = ctx->flag;
+  d.rho = ctx->rho;
+  d.vx = ctx->vx;
+  d.vy = ctx->vy;
+  d.vz = ctx->vz;
+  if (ctx->inplace) {
+    d.a.src = ctx->buf[0];
+    if (ctx->parity == 1)
+      hipLaunchKernelGGL((k_dump<S, 1>), dim3(d.a.nblk), dim3(kBlock), 0, ctx->stream, d);
+    else
+      hipLaunchKernelGGL((k_dump<S, 2>), dim3(d.a.nblk), dim3(kBlock), 0, ctx->stream, d);
+  } else {
+    d.a.src = ctx->buf[ctx->cur];
+    hipLaunchKernelGGL((k_dump<S, 0>), dim3(d.a.nblk), dim3(kBlock), 0, ctx->stream, d);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return CULBM_OK;
+}
+
+template <class F>
+int by_storage(int32_t storage, F&& f) {
+  switch (storage) {
+    case CULBM_STORAGE_FP16: return f(StoreF16{});
+    case CULBM_STORAGE_BF16: return f(StoreBF16{});
+    default: return f(StoreF32{});
+  }
+}
+
+int zero_device(culbm_ctx* ctx, void* p, size_t bytes) {
+  const uint64_t words = bytes / 4;
+  const uint32_t nblk = uint32_t(std::min<uint64_t>((words + kBlock - 1) / kBlock, 65536));
+  if (words)
+    hipLaunchKernelGGL(k_zero, dim3(nblk), dim3(kBlock), 0, ctx->stream,
+                       static_cast<uint32_t*>(p), words);
+  HIPCHK(ctx, hipGetLastError());
+  return CULBM_OK;
+}
+
+size_t ddf_bytes(const culbm_ctx* ctx) { return size_t(27) * size_t(ctx->dstride) * size_t(ctx->bpp); }
+
+int alloc(culbm_ctx* ctx, void** p, size_t bytes) {
+  HIPCHK(ctx, hipMalloc(p, bytes));
+  ctx->device_bytes += int64_t(bytes);
+  return CULBM_OK;
+}
+
+// Planes per wavefront chunk: the chunk's working set (D + T + 1 planes of
+// every buffer plus flags) within ~160 MB of the 256 MB Infinity Cache.
+int32_t auto_depth(const culbm_ctx* ctx) {
+  const double plane = double(ctx->cfg.nx) * ctx->cfg.ny *
+                       (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
+  const double budget = 160.0 * 1024 * 1024;
+  const int32_t T = std::max(1, ctx->cfg.inner_loop);
+  int32_t d = int32_t(budget / plane) - T - 1;
+  d = std::max(d, 2);
+  return std::min(d, ctx->cfg.nz);
+}
+
+}  // namespace
+
+extern "C" {
+
+int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
+  if (!out) return fail(nullptr, CULBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  char msg[512];
+  const int v = culbm_validate(cfg, msg, sizeof(msg));
+  if (v) return fail(nullptr, v, "%s", msg);
+
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, CULBM_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (cfg->device >= ndev)
+    return fail(nullptr, CULBM_ERR_INVALID, "devId %d out of range (%d devices)", cfg->device, ndev);
+
+  culbm_ctx* ctx = new culbm_ctx();
+  ctx->cfg = *cfg;
+  auto bail = [&](int rc) {
+    g_create_err = ctx->err;
+    culbm_destroy(ctx);
+    return rc;
+  };
+#define CK(expr)                   \
+  do {                             \
+    const int rc_ = (expr);        \
+    if (rc_) return bail(rc_);     \
+  } while (0)
+#define HK(expr)                                                                    \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      fail(ctx, CULBM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+      return bail(CULBM_ERR_HIP);                                                   \
+    }                                                                               \
+  } while (0)
+
+  HK(hipSetDevice(cfg->device));
+  HK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HK(hipEventCreate(&ctx->ev0));
+  HK(hipEventCreate(&ctx->ev1));
+
+  const culbm_config& c = ctx->cfg;
+  ctx->n = int64_t(c.nx) * c.ny * c.nz;
+  ctx->inplace = c.stream_policy == CULBM_STREAM_INPLACE;
+  ctx->bpp = c.storage == CULBM_STORAGE_FP32 ? 4 : 2;
+  ctx->scale = c.storage == CULBM_STORAGE_FP32 ? 1.0f
+               : (c.storage_scale != 0.0f ? c.storage_scale : default_scale(c.storage));
+  ctx->inv_scale = 1.0f / ctx->scale;
+  if (ctx->inplace) {
+    ctx->px = int32_t(round_up(int64_t(c.nx) + 1, 128 / ctx->bpp));
+    ctx->py = c.ny + 1;
+    ctx->pz = c.nz + 1;
+  } else {
+    ctx->px = c.nx;
+    ctx->py = c.ny;
+    ctx->pz = c.nz;
+  }
+  ctx->dstride = round_up(int64_t(ctx->px) * ctx->py * ctx->pz, 256);
+
+  const size_t nb = size_t(ctx->n);
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->flag), nb * 4));
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->sflag), round_up(int64_t(nb), 4)));
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->rho), nb * 4));
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->vx), nb * 4));
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->vy), nb * 4));
+  CK(alloc(ctx, reinterpret_cast<void**>(&ctx->vz), nb * 4));
+  CK(alloc(ctx, &ctx->buf[0], ddf_bytes(ctx)));
+  if (!ctx->inplace) CK(alloc(ctx, &ctx->buf[1], ddf_bytes(ctx)));
+
+  if (blocked(c)) {
+    ctx->wave_depth = c.wave_depth > 0 ? std::min(c.wave_depth, c.nz) : auto_depth(ctx);
+    if (c.opt_policy == CULBM_OPT_STATIC) {
+      int per_cu = 0;
+      const void* fn = nullptr;
+      by_storage(c.storage, [&](auto s) {
+        using S = decltype(s);
+        fn = ctx->inplace ? reinterpret_cast<const void*>(&k_wave_coop<S, true>)
+                          : reinterpret_cast<const void*>(&k_wave_coop<S, false>);
+        return 0;
+      });
+      HK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+      hipDeviceProp_t prop;
+      HK(hipGetDeviceProperties(&prop, c.device));
+      if (per_cu < 1) {
+        fail(ctx, CULBM_ERR_UNSUPPORTED, "cooperative wavefront kernel cannot be resident");
+        return bail(CULBM_ERR_UNSUPPORTED);
+      }
+      ctx->coop_blocks = per_cu * prop.multiProcessorCount;
+    }
+  }
+
+  // reference defaults: all FLUID + DUMP, rho 1, u 0, rest populations
+  {
+    std::vector<uint32_t> fl(nb, CULBM_FLAG_LOAD | CULBM_FLAG_COLLIDE | CULBM_FLAG_STORE |
+                                     CULBM_FLAG_DUMP_RHO | CULBM_FLAG_DUMP_VX |
+                                     CULBM_FLAG_DUMP_VY | CULBM_FLAG_DUMP_VZ);
+    CK(culbm_set_state(ctx, fl.data(), nullptr, nullptr, nullptr, nullptr));
+  }
+  CK(culbm_init_ddf(ctx, CULBM_INIT_REST));
+#undef CK
+#undef HK
+  *out = ctx;
+  return CULBM_OK;
+}
+
+int culbm_destroy(culbm_ctx* ctx) {
+  if (!ctx) return CULBM_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  void* ptrs[] = {ctx->buf[0], ctx->buf[1], ctx->flag, ctx->sflag, ctx->rho, ctx->vx, ctx->vy, ctx->vz};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CULBM_OK;
+}
+
+int culbm_set_state(culbm_ctx* ctx, const uint32_t* flag, const float* rho, const float* vx,
+                    const float* vy, const float* vz) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t nb = size_t(ctx->n);
+  if (flag) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->flag, flag, nb * 4, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t nblk = uint32_t((nb + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_compact_flags, dim3(nblk), dim3(kBlock), 0, ctx->stream, ctx->flag,
+                       ctx->sflag, uint64_t(nb));
+    HIPCHK(ctx, hipGetLastError());
+  }
+  const float* src[4] = {rho, vx, vy, vz};
+  float* dst[4] = {ctx->rho, ctx->vx, ctx->vy, ctx->vz};
+  const float dflt[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+  std::vector<float> fill;
+  for (int k = 0; k < 4; ++k) {
+    if (src[k]) {
+      HIPCHK(ctx, hipMemcpyAsync(dst[k], src[k], nb * 4, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+      if (fill.empty() || fill[0] != dflt[k]) fill.assign(nb, dflt[k]);
+      HIPCHK(ctx, hipMemcpyAsync(dst[k], fill.data(), nb * 4, hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // `fill` is reused
+    }
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CULBM_OK;
+}
+
+int culbm_init_ddf(culbm_ctx* ctx, int32_t mode) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (mode != CULBM_INIT_REST && mode != CULBM_INIT_FEQ)
+    return fail(ctx, CULBM_ERR_INVALID, "init mode must be 0 (rest) or 1 (feq), got %d", mode);
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  for (int b = 0; b < 2; ++b)
+    if (ctx->buf[b]) {
+      const int rc = zero_device(ctx, ctx->buf[b], ddf_bytes(ctx));
+      if (rc) return rc;
+    }
+  ctx->cur = 0;
+  ctx->parity = 1;
+  if (mode == CULBM_INIT_FEQ) {
+    float* plain = nullptr;
+    HIPCHK(ctx, hipMalloc(&plain, size_t(27) * size_t(ctx->n) * 4));
+    StepArgs a = full_args(ctx);
+    hipLaunchKernelGGL(k_feq_plain, dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, plain);
+    int rc = CULBM_OK;
+    if (hipGetLastError() != hipSuccess) rc = fail(ctx, CULBM_ERR_HIP, "feq kernel launch failed");
+    if (!rc) rc = by_storage(ctx->cfg.storage, [&](auto s) { return upload_plain<decltype(s)>(ctx, plain); });
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(plain);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->step = 0;
+  return CULBM_OK;
+}
+
+int culbm_step(culbm_ctx* ctx, int32_t nsteps) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (nsteps < 0) return fail(ctx, CULBM_ERR_INVALID, "nsteps must be >= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  const int rc = steps_dispatch(ctx, nsteps);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CULBM_OK;
+}
+
+int culbm_time_steps(culbm_ctx* ctx, int32_t nsteps, float* ms) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (nsteps < 0) return fail(ctx, CULBM_ERR_INVALID, "nsteps must be >= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  const int rc = steps_dispatch(ctx, nsteps);
+  if (rc) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+  float t = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  if (ms) *ms = t;
+  return CULBM_OK;
+}
+
+int culbm_profile_steps(culbm_ctx* ctx, int32_t nsteps, float* kernel_ms, int32_t* launches) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (nsteps < 0) return fail(ctx, CULBM_ERR_INVALID, "nsteps must be >= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  ctx->prof = true;
+  ctx->prof_used = 0;
+  const int rc = steps_dispatch(ctx, nsteps);
+  ctx->prof = false;
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double total = 0.0;
+  for (size_t k = 0; k < ctx->prof_used; ++k) {
+    float t = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&t, ctx->prof_ev[2 * k], ctx->prof_ev[2 * k + 1]));
+    total += t;
+  }
+  if (kernel_ms) *kernel_ms = float(total);
+  if (launches) *launches = int32_t(ctx->prof_used);
+  return CULBM_OK;
+}
+
+int culbm_run_window(culbm_ctx* ctx, int32_t dstep, int32_t* steps_done, float* ms) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (dstep < 0) return fail(ctx, CULBM_ERR_INVALID, "dstep must be >= 0");
+  int32_t steps = dstep;
+  if (blocked(ctx->cfg)) {
+    const int32_t I = ctx->cfg.inner_loop;
+    steps = (dstep + I - 1) / I * I;
+  }
+  const int rc = culbm_time_steps(ctx, steps, ms);
+  if (rc) return rc;
+  if (steps_done) *steps_done = steps;
+  return CULBM_OK;
+}
+
+int culbm_dump(culbm_ctx* ctx) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  const int rc = by_storage(ctx->cfg.storage, [&](auto s) { return dump_impl<decltype(s)>(ctx); });
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CULBM_OK;
+}
+
+int culbm_get_macros(culbm_ctx* ctx, float* rho, float* vx, float* vy, float* vz) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  float* dst[4] = {rho, vx, vy, vz};
+  const float* src[4] = {ctx->rho, ctx->vx, ctx->vy, ctx->vz};
+  for (int k = 0; k < 4; ++k)
+    if (dst[k])
+      HIPCHK(ctx, hipMemcpyAsync(dst[k], src[k], size_t(ctx->n) * 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CULBM_OK;
+}
+
+int culbm_get_ddf(culbm_ctx* ctx, float* ddf) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (!ddf) return fail(ctx, CULBM_ERR_INVALID, "ddf is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t bytes = size_t(27) * size_t(ctx->n) * 4;
+  float* plain = nullptr;
+  HIPCHK(ctx, hipMalloc(&plain, bytes));
+  int rc = by_storage(ctx->cfg.storage, [&](auto s) { return download_plain<decltype(s)>(ctx, plain); });
+  if (!rc && hipMemcpyAsync(ddf, plain, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = fail(ctx, CULBM_ERR_HIP, "device-to-host copy failed");
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(plain);
+  return rc;
+}
+
+int culbm_set_ddf(culbm_ctx* ctx, const float* ddf) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (!ddf) return fail(ctx, CULBM_ERR_INVALID, "ddf is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  const size_t bytes = size_t(27) * size_t(ctx->n) * 4;
+  float* plain = nullptr;
+  HIPCHK(ctx, hipMalloc(&plain, bytes));
+  int rc = CULBM_OK;
+  if (hipMemcpyAsync(plain, ddf, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = fail(ctx, CULBM_ERR_HIP, "host-to-device copy failed");
+  if (!rc) {
+    for (int b = 0; b < 2 && !rc; ++b)
+      if (ctx->buf[b]) rc = zero_device(ctx, ctx->buf[b], ddf_bytes(ctx));
+  }
+  if (!rc) rc = by_storage(ctx->cfg.storage, [&](auto s) { return upload_plain<decltype(s)>(ctx, plain); });
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(plain);
+  return rc;
+}
+
+int culbm_get_info(culbm_ctx* ctx, culbm_info* info) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (!info) return fail(ctx, CULBM_ERR_INVALID, "info is NULL");
+  std::memset(info, 0, sizeof(*info));
+  info->step = ctx->step;
+  info->device_bytes = ctx->device_bytes;
+  info->wave_depth = ctx->wave_depth;
+  info->bytes_per_pop = ctx->bpp;
+  info->storage_scale = ctx->scale;
+  const culbm_config& c = ctx->cfg;
+  if (!blocked(c)) {
+    info->launches_per_pass = 1;
+  } else if (c.opt_policy == CULBM_OPT_STATIC) {
+    info->launches_per_pass = 1;
+  } else {
+    const int32_t T = c.inner_loop, D = ctx->wave_depth, nz = c.nz;
+    const int32_t K = (nz + T - 1 + D - 1) / D;
+    int32_t cnt = 0;
+    for (int32_t k = 0; k < K; ++k)
+      for (int32_t t = 1; t <= T; ++t)
+        if (std::max(0, k * D - (t - 1)) < std::min(nz, (k + 1) * D - (t - 1))) ++cnt;
+    info->launches_per_pass = cnt;
+  }
+  return CULBM_OK;
+}
+
+// ---- probes -----------------------------------------------------------------
+
+int culbm_probe_cells(int32_t device, int32_t op, float inv_tau, float* f, const float* in4,
+                      float* out4, int64_t n) {
+  if (n <= 0 || n > (int64_t(1) << 26)) return fail(nullptr, CULBM_ERR_INVALID, "bad n");
+  if (op < 0 || op > 2) return fail(nullptr, CULBM_ERR_INVALID, "bad op");
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CULBM_ERR_HIP, "hipSetDevice failed");
+  float *df = nullptr, *din = nullptr, *dout = nullptr;
+  int rc = CULBM_OK;
+  const size_t fb = size_t(n) * 27 * 4, qb = size_t(n) * 4 * 4;
+  if (hipMalloc(&df, fb) != hipSuccess || hipMalloc(&din, qb) != hipSuccess ||
+      hipMalloc(&dout, qb) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "allocation failed");
+  if (!rc && op != 1 && hipMemcpy(df, f, fb, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (!rc && op == 1 && hipMemcpy(din, in4, qb, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (!rc) {
+    hipLaunchKernelGGL(k_cell_probe, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0,
+                       op, inv_tau, df, din, df, dout, uint32_t(n));
+    if (hipDeviceSynchronize() != hipSuccess) rc = fail(nullptr, CULBM_ERR_HIP, "probe kernel failed");
+  }
+  if (!rc && op != 2 && hipMemcpy(f, df, fb, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (!rc && op == 2 && hipMemcpy(out4, dout, qb, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (df) (void)hipFree(df);
+  if (din) (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  return rc;
+}
+
+int culbm_probe_codec(int32_t device, int32_t storage, float scale, const float* in,
+                      uint16_t* enc, float* dec, int64_t n) {
+  if (n <= 0 || n > (int64_t(1) << 28)) return fail(nullptr, CULBM_ERR_INVALID, "bad n");
+  if (storage < 0 || storage > 2) return fail(nullptr, CULBM_ERR_INVALID, "bad storage");
+  if (!(scale > 0.0f)) return fail(nullptr, CULBM_ERR_INVALID, "bad scale");
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CULBM_ERR_HIP, "hipSetDevice failed");
+  float *din = nullptr, *ddec = nullptr;
+  uint16_t* denc = nullptr;
+  int rc = CULBM_OK;
+  if (hipMalloc(&din, size_t(n) * 4) != hipSuccess || hipMalloc(&ddec, size_t(n) * 4) != hipSuccess ||
+      hipMalloc(&denc, size_t(n) * 2) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "allocation failed");
+  if (!rc && hipMemcpy(din, in, size_t(n) * 4, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (!rc) {
+    const dim3 g(uint32_t((n + kBlock - 1) / kBlock));
+    const float inv = 1.0f / scale;
+    if (storage == 1)
+      hipLaunchKernelGGL(k_codec<StoreF16>, g, dim3(kBlock), 0, 0, din, denc, ddec, uint32_t(n), scale, inv);
+    else if (storage == 2)
+      hipLaunchKernelGGL(k_codec<StoreBF16>, g, dim3(kBlock), 0, 0, din, denc, ddec, uint32_t(n), scale, inv);
+    else
+      hipLaunchKernelGGL(k_codec<StoreF32>, g, dim3(kBlock), 0, 0, din, denc, ddec, uint32_t(n), scale, inv);
+    if (hipDeviceSynchronize() != hipSuccess) rc = fail(nullptr, CULBM_ERR_HIP, "codec kernel failed");
+  }
+  if (!rc && enc && storage != 0 && hipMemcpy(enc, denc, size_t(n) * 2, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (!rc && dec && hipMemcpy(dec, ddec, size_t(n) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "copy failed");
+  if (din) (void)hipFree(din);
+  if (ddec) (void)hipFree(ddec);
+  if (denc) (void)hipFree(denc);
+  return rc;
+}
+
+int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs) {
+  if (bytes < 4096 || bytes > (int64_t(64) << 30) || iters < 1 || mode < 0 || mode > 4)
+    return fail(nullptr, CULBM_ERR_INVALID, "bad bandwidth probe arguments");
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CULBM_ERR_HIP, "hipSetDevice failed");
+  const uint64_t n4 = uint64_t(bytes) / 16;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = CULBM_OK;
+  if (hipMalloc(&a, n4 * 16 + 256) != hipSuccess || hipMalloc(&b, n4 * 16 + 256) != hipSuccess)
+    rc = fail(nullptr, CULBM_ERR_HIP, "allocation failed");
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(nullptr, CULBM_ERR_HIP, "event creation failed");
+  if (!rc) {
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, device);
+    const uint32_t nblk = uint32_t(prop.multiProcessorCount) * 16;
+    hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(a), n4);
+    hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4);
+    auto launch = [&]() {
+      switch (mode) {
+        case 0: hipLaunchKernelGGL(k_bw_copy, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float4*>(b), n4); break;
+        case 1: hipLaunchKernelGGL(k_bw_read, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float*>(b), n4); break;
+        case 2: hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4); break;
+        case 3: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 0u); break;
+        default: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 1u); break;
+      }
+    };
+    launch();  // warm-up
+    (void)hipEventRecord(e0, 0);
+    for (int32_t k = 0; k < iters; ++k) launch();
+    (void)hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(nullptr, CULBM_ERR_HIP, "probe kernel failed");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double moved = double(n4) * 16.0 * (mode == 0 || mode >= 3 ? 2.0 : 1.0) * iters;
+    if (gbs && ms > 0.0f) *gbs = float(moved / (double(ms) * 1e-3) / 1e9);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Kernel-side data structures shared by the step, dump and layout kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_physics.hpp"
+
+namespace culbm {
+
+// Unsigned 32-bit division by a run-time invariant (round-up multiplier):
+// q = (mulhi(m, n) + n) >> l, evaluated in 64 bits; exact for all n < 2^32.
+struct FastDiv {
+  uint32_t d, m, l;
+  __host__ static FastDiv make(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    f.l = l;
+    f.m = uint32_t(((((uint64_t(1) << l) - d) << 32) / d) + 1);
+    return f;
+  }
+  __host__ __device__ uint32_t div(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t t = uint64_t(__umulhi(m, n)) + n;
+#else
+    const uint64_t t = ((uint64_t(m) * n) >> 32) + n;
+#endif
+    return uint32_t(t >> l);
+  }
+};
+
+// Storage formats of the populations.  All three hold the shifted value
+// f_i - w_i; the 16-bit ones hold it multiplied by a power-of-two scale, so
+// encode = round-to-nearest-even(scale * f) and decode = h * (1/scale) are
+// exact apart from the single rounding.
+enum Storage : int32_t { kFP32 = 0, kFP16 = 1, kBF16 = 2 };
+
+struct StoreF32 {
+  using T = float;
+  static constexpr int kBytes = 4;
+  __device__ __forceinline__ static float dec(T v, float) { return v; }
+  __device__ __forceinline__ static T enc(float v, float) { return v; }
+};
+
+struct StoreF16 {
+  using T = _Float16;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float dec(T v, float inv_s) { return float(v) * inv_s; }
+  // `+ 0.0f` maps -0 to +0 before rounding: the compiler may fold the scale
+  // and the conversion into one mixed-precision FMA with a +0 addend, which
+  // does the same; writing it out makes every code path agree.
+  __device__ __forceinline__ static T enc(float v, float s) { return T(v * s + 0.0f); }
+};
+
+struct StoreBF16 {
+  using T = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float dec(T v, float inv_s) {
+    return __uint_as_float(uint32_t(v) << 16) * inv_s;
+  }
+  __device__ __forceinline__ static T enc(float v, float s) {
+    uint32_t u = __float_as_uint(v * s);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return T(u >> 16);
+  }
+};
+
+// Arguments of one step launch over the z-plane range [z0, z1).
+//
+// Domain arrays (flags, rho/u) are dense [nz][ny][nx].  Population storage is
+// [27][pz][py][px] with direction stride `dstride`; the pull layout uses
+// px=nx, py=ny, pz=nz, the in-place layout pads every axis by one cell so the
+// twisted store at x+1 of the last cell has a home.
+struct StepArgs {
+  int32_t nx, ny, nz;
+  int32_t px, py;        // layout pitches (cells per row, rows per plane)
+  int32_t z0, z1;        // plane range of this launch
+  uint32_t ncells;       // (z1-z0)*nx*ny
+  int64_t dstride;       // elements between consecutive directions
+  FastDiv divx, divy;    // divide by nx, by ny
+  uint32_t nblk;         // workgroups in the launch (for the XCD remap)
+  const uint8_t* sflag;  // per-cell step bits (kStepBits of the flag word)
+  const float* rho;
+  const float* vx;
+  const float* vy;
+  const float* vz;
+  const void* src;
+  void* dst;  // pull: destination buffer; in-place: same as src
+  float inv_tau, omt;
+  float scale, inv_scale;
+};
+
+}  // namespace culbm

@@ -1,0 +1,195 @@
+/*
+ * culbm.h - C ABI of the MI355X D3Q27 lattice-Boltzmann engine.
+ *
+ * The reference exposes no library API: its boundary is the process
+ * (culbm::simulator::single_dev_expt::Simulator(argc, argv), run(), ~Simulator(),
+ * src/simulator/single_dev_expt/simulator_expt_platform.hpp:7-16, driven by
+ * src/single_dev_expt_main.cpp:5-17).  These entry points split that object
+ * into the calls its constructor and run() make, so a host CLI, ctypes or any
+ * FFI can drive the same path:
+ *
+ *   culbm_create        Simulator::Data::Data, buffer allocation
+ *                       (simulator_expt_platform.cu:308-445)
+ *   culbm_set_state     initFlag / initRhoU (simulator_expt_platform.cu:95-118, 200-228)
+ *   culbm_init_ddf      initDoubleDDFBuf / initRingDDFBuf (simulator_expt_platform.cu:230-277)
+ *   culbm_step          the step launches of the policy runs (simulator_expt_platform.cu:749-991)
+ *   culbm_run_window    one timed window of run() (event record, dstep steps,
+ *                       MLUPS), simulator_expt_platform.cu:963-985
+ *   culbm_dump          the *DumpKernel launches (kernel.cu:2634-2735, 1829-1936)
+ *   culbm_get_macros    dumpRes' device-to-host copies (simulator_expt_platform.cu:572-606)
+ *   culbm_destroy       Simulator::Data::~Data (simulator_expt_platform.cu:447-502)
+ *
+ * Conventions: every call returns CULBM_OK (0) or a CULBM_ERR_* code and never
+ * throws; culbm_last_error() gives the message.  The library owns all device
+ * memory; callers own host arrays.  Host arrays are dense x-fastest
+ * [nz][ny][nx] (populations: [27][nz][ny][nx]), little-endian, exactly the
+ * layout of the reference's .dat files.  One context owns one HIP stream and is
+ * not thread safe.
+ */
+#ifndef CULBM_H
+#define CULBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CULBM_ABI_VERSION 1
+
+/* return codes */
+#define CULBM_OK 0
+#define CULBM_ERR_INVALID 1     /* bad argument or configuration              */
+#define CULBM_ERR_HIP 2         /* HIP runtime error                          */
+#define CULBM_ERR_STATE 3       /* call not valid in the current state        */
+#define CULBM_ERR_UNSUPPORTED 4 /* combination not provided                   */
+
+/* flag bits (src/simulator/single_dev_expt/local_config.hpp:12-22) */
+#define CULBM_FLAG_LOAD (1u << 0)
+#define CULBM_FLAG_EQU (1u << 1)
+#define CULBM_FLAG_COLLIDE (1u << 2)
+#define CULBM_FLAG_BOUNCE_BACK (1u << 3)
+#define CULBM_FLAG_STORE (1u << 4)
+#define CULBM_FLAG_DUMP_RHO (1u << 27)
+#define CULBM_FLAG_DUMP_VX (1u << 28)
+#define CULBM_FLAG_DUMP_VY (1u << 29)
+#define CULBM_FLAG_DUMP_VZ (1u << 30)
+
+/* --streamPolicy */
+#define CULBM_STREAM_PULL 0
+#define CULBM_STREAM_INPLACE 1
+/* --optPolicy */
+#define CULBM_OPT_NONE 0
+#define CULBM_OPT_STATIC 1  /* reference: HALO_BLOCKING_STATIC_L2  */
+#define CULBM_OPT_MIXED 2   /* reference: HALO_BLOCKING_L1L2       */
+#define CULBM_OPT_DYNAMIC 3 /* reference: HALO_BLOCKING_DYNAMIC_L2 */
+/* population storage (no reference counterpart: the reference stores FP32) */
+#define CULBM_STORAGE_FP32 0
+#define CULBM_STORAGE_FP16 1
+#define CULBM_STORAGE_BF16 2
+/* culbm_init_ddf modes */
+#define CULBM_INIT_REST 0 /* feq(1,0) = 0 everywhere, as the reference   */
+#define CULBM_INIT_FEQ 1  /* feq(rho, u) of the rho/u fields (opt-in)     */
+
+typedef struct culbm_ctx culbm_ctx;
+
+/* Knobs of the reference CLI (simulator_expt_platform.cu:312-354) plus the
+ * storage format and the wavefront depth of the blocked policies. */
+typedef struct culbm_config {
+  int32_t nx, ny, nz;       /* --domainDim                                   */
+  float inv_tau;            /* --invTau                                      */
+  int32_t stream_policy;    /* --streamPolicy                                */
+  int32_t opt_policy;       /* --optPolicy                                   */
+  int32_t inner_loop;       /* --innerLoop: steps per temporal-blocking pass */
+  int32_t block_dim[3];     /* --blockDim (tile-shape knob, see DESIGN)      */
+  int32_t grid_dim[3];      /* --gridDim                                     */
+  int32_t use_ring;         /* --useRingDDFBuf                               */
+  int32_t device;           /* --devId                                       */
+  int32_t storage;          /* CULBM_STORAGE_*                               */
+  float storage_scale;      /* 16-bit scale (power of two); 0 = default      */
+  int32_t wave_depth;       /* planes per wavefront chunk; 0 = automatic     */
+  int32_t reserved[7];
+} culbm_config;
+
+typedef struct culbm_info {
+  int64_t step;               /* cumulative steps taken                      */
+  int64_t device_bytes;       /* bytes of device memory held                 */
+  int32_t wave_depth;         /* chunk depth in use (blocked policies)       */
+  int32_t launches_per_pass;  /* kernel launches per pass of inner_loop steps */
+  int32_t bytes_per_pop;      /* 4 (FP32) or 2 (FP16/BF16)                   */
+  float storage_scale;        /* scale in use                                */
+  int32_t reserved[8];
+} culbm_info;
+
+/* Fill cfg with the reference CLI defaults (simulator_expt_platform.cu:312-354). */
+void culbm_config_default(culbm_config* cfg);
+
+/* Validate a configuration without touching the device.  Returns CULBM_OK or
+ * an error code; msg (may be NULL) receives the reason. */
+int culbm_validate(const culbm_config* cfg, char* msg, size_t msg_len);
+
+/* Validate, select the device, allocate buffers.  Flags default to the
+ * reference's no-init-folder state (all FLUID + DUMP), rho = 1, u = 0, and
+ * the populations to the rest equilibrium. */
+int culbm_create(const culbm_config* cfg, culbm_ctx** out);
+
+/* Upload the flag field and the rho/vx/vy/vz fields (host arrays of nx*ny*nz
+ * values).  flag == NULL keeps the current flags; a NULL field is reset to its
+ * default (rho 1, u 0), as initRhoU does for a missing file. */
+int culbm_set_state(culbm_ctx* ctx, const uint32_t* flag, const float* rho, const float* vx,
+                    const float* vy, const float* vz);
+
+/* (Re)initialise the populations: CULBM_INIT_REST or CULBM_INIT_FEQ.  Resets
+ * the step counter. */
+int culbm_init_ddf(culbm_ctx* ctx, int32_t mode);
+
+/* Advance exactly nsteps steps (any policy; blocked policies run passes of
+ * inner_loop steps and a shorter final pass). */
+int culbm_step(culbm_ctx* ctx, int32_t nsteps);
+
+/* One window of the reference's run(): time the step launches of `dstep`
+ * steps with device events.  Blocked policies advance in whole passes, so
+ * they take ceil(dstep / inner_loop) * inner_loop steps, as the reference's
+ * window loop does (simulator_expt_platform.cu:785).  Outputs may be NULL. */
+int culbm_run_window(culbm_ctx* ctx, int32_t dstep, int32_t* steps_done, float* ms);
+
+/* Device-timed run of nsteps steps (event pair on the context stream around
+ * the launches only); *ms receives the elapsed milliseconds. */
+int culbm_time_steps(culbm_ctx* ctx, int32_t nsteps, float* ms);
+
+/* Device-timed run of nsteps steps with an event pair around every kernel
+ * launch: *kernel_ms is the summed kernel time (launch gaps excluded),
+ * *launches the number of launches.  Measurement aid; slower than
+ * culbm_time_steps. */
+int culbm_profile_steps(culbm_ctx* ctx, int32_t nsteps, float* kernel_ms, int32_t* launches);
+
+/* Dump kernel: rho/u of the current populations written into the device
+ * rho/vx/vy/vz buffers where the DUMP bits are set. */
+int culbm_dump(culbm_ctx* ctx);
+
+/* Copy the device rho/vx/vy/vz buffers to host arrays (NULL skips a field). */
+int culbm_get_macros(culbm_ctx* ctx, float* rho, float* vx, float* vy, float* vz);
+
+/* Populations as dense FP32 [27][nz][ny][nx], post-collision view of the last
+ * step (decoded from 16-bit storage). */
+int culbm_get_ddf(culbm_ctx* ctx, float* ddf);
+
+/* Replace the populations (same layout as culbm_get_ddf; encoded to the
+ * storage format).  The step counter is kept. */
+int culbm_set_ddf(culbm_ctx* ctx, const float* ddf);
+
+int culbm_get_info(culbm_ctx* ctx, culbm_info* info);
+
+/* Release device memory and the stream.  NULL is accepted. */
+int culbm_destroy(culbm_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of the last failing
+ * culbm_create when ctx is NULL).  Never NULL. */
+const char* culbm_last_error(const culbm_ctx* ctx);
+
+/* Name of a return code. */
+const char* culbm_error_string(int code);
+
+/* ---- test probes (single cells, codecs); device arrays are managed here ---- */
+
+/* op 0: collide f[n][27] in place; op 1: equilibrium of in4[n][4] = (rho,u)
+ * into f; op 2: moments of f into out4[n][4]. */
+int culbm_probe_cells(int32_t device, int32_t op, float inv_tau, float* f, const float* in4,
+                      float* out4, int64_t n);
+
+/* Encode+decode n floats through a storage format on the device. */
+int culbm_probe_codec(int32_t device, int32_t storage, float scale, const float* in,
+                      uint16_t* enc, float* dec, int64_t n);
+
+/* Bandwidth calibration (counterpart of the reference's analysis/lambda and
+ * analysis/gamma probes): mode 0 copy, 1 read, 2 write (16-byte accesses),
+ * 3 scalar copy, 4 scalar copy with every store shifted by one element.
+ * `bytes` per buffer; *gbs = bytes moved / time. */
+int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CULBM_H */

@@ -1,0 +1,456 @@
+// CPU oracle for the D3Q27 BGK collide+stream path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: tests, the package's
+// smoke() and bench.py's CPU-baseline leg may load it (as liboracle.so);
+// the product library never does.
+//
+// It restates, in plain host code, the reference update of cuLBM's
+// single-device engine:
+//   - velocity set and weights           src/basic/velocity_set.hpp:189-255
+//   - shifted equilibrium calcEqu<27>    src/lbm_core/device_function.hpp:305-398
+//   - moments calcRhoU<27>               src/lbm_core/device_function.hpp:439-463
+//   - BGK collision2<27>                 src/lbm_core/device_function.hpp:676-754
+//   - flag bits                          src/simulator/single_dev_expt/local_config.hpp:12-22
+//   - pull step with face clamp          src/simulator/single_dev_expt/kernel.cu:2489-2632
+//   - dump of rho/u                      src/simulator/single_dev_expt/kernel.cu:2634-2735
+//   - halo-blocking axis math            src/blocking_core/blocking_alogrithm.hpp:8-41
+//
+// Two arithmetic modes:
+//   * FP64: the same algebra evaluated in double.  This is the accuracy oracle.
+//   * FP32: the reference's exact single-precision operation order, compiled
+//     with -ffp-contract=off, so results are reproducible bit for bit.  The
+//     FP32 lattice step is OpenMP-parallel over (z, y) rows and doubles as
+//     the CPU baseline.
+// 16-bit shifted storage (FP16 / BF16 with a power-of-two scale) has no
+// reference counterpart; the functions here define it.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <omp.h>
+
+namespace {
+
+constexpr uint32_t F_LOAD = 1u << 0;
+constexpr uint32_t F_EQU = 1u << 1;
+constexpr uint32_t F_COLLIDE = 1u << 2;
+constexpr uint32_t F_BB = 1u << 3;
+constexpr uint32_t F_STORE = 1u << 4;
+constexpr uint32_t F_DUMP_RHO = 1u << 27;
+constexpr uint32_t F_DUMP_VX = 1u << 28;
+constexpr uint32_t F_DUMP_VY = 1u << 29;
+constexpr uint32_t F_DUMP_VZ = 1u << 30;
+
+// Direction i = (cx+1) + 3(cy+1) + 9(cz+1); opposite = 26 - i; rest = 13.
+inline int cx_of(int i) { return i % 3 - 1; }
+inline int cy_of(int i) { return (i / 3) % 3 - 1; }
+inline int cz_of(int i) { return i / 9 - 1; }
+
+// Equilibrium deviation feq_i - w_i for one cell (calcEqu<27>).
+// u' = 3u, c3 = -3|u|^2; non-rest: fma(w rho, fma(1/2, fma(s, s, c3), s), w (rho-1))
+// with s = c_i . u' formed from the same pairwise sums as the reference;
+// rest: w0 * fma(rho, c3/2, rho-1).
+template <typename R>
+struct Eq {
+  R s[27];    // signed projection c_i . u' (s[13] unused)
+  R c3, rho, rhom1;
+};
+
+template <typename R>
+inline void eq_terms(R rho, R vx, R vy, R vz, Eq<R>& e) {
+  e.rho = rho;
+  e.rhom1 = rho - R(1);
+  e.c3 = R(-3) * (vx * vx + vy * vy + vz * vz);
+  vx *= R(3);
+  vy *= R(3);
+  vz *= R(3);
+  const R v0 = vx + vy, v1 = vx + vz, v2 = vy + vz, v3 = vx - vy, v4 = vx - vz,
+          v5 = vy - vz, v6 = vx + vy + vz, v7 = vx + vy - vz, v8 = vx - vy + vz,
+          v9 = vy - vx + vz;
+  R* s = e.s;
+  s[0] = -v6;  s[1] = -v2;  s[2] = -v9;
+  s[3] = -v1;  s[4] = -vz;  s[5] = v4;
+  s[6] = -v8;  s[7] = v5;   s[8] = v7;
+  s[9] = -v0;  s[10] = -vy; s[11] = v3;
+  s[12] = -vx; s[13] = R(0); s[14] = vx;
+  s[15] = -v3; s[16] = vy;  s[17] = v0;
+  s[18] = -v7; s[19] = -v5; s[20] = v8;
+  s[21] = -v4; s[22] = vz;  s[23] = v1;
+  s[24] = v9;  s[25] = v2;  s[26] = v6;
+}
+
+template <typename R>
+struct W {
+  static constexpr R W0 = R(8) / R(27);
+  static constexpr R WS = R(2) / R(27);
+  static constexpr R WE = R(1) / R(54);
+  static constexpr R WC = R(1) / R(216);
+};
+
+// weight class per direction: 0 rest, 1 face (straight), 2 edge, 3 corner
+inline int wclass(int i) {
+  const int n = (cx_of(i) != 0) + (cy_of(i) != 0) + (cz_of(i) != 0);
+  return n;
+}
+
+template <typename R>
+inline void feq_from(const Eq<R>& e, R* feq) {
+  const R rhos = W<R>::WS * e.rho, rhoe = W<R>::WE * e.rho, rhoc = W<R>::WC * e.rho;
+  const R rhom1s = W<R>::WS * e.rhom1, rhom1e = W<R>::WE * e.rhom1,
+          rhom1c = W<R>::WC * e.rhom1;
+  for (int i = 0; i < 27; ++i) {
+    if (i == 13) continue;
+    const int k = wclass(i);
+    const R rw = k == 1 ? rhos : (k == 2 ? rhoe : rhoc);
+    const R mw = k == 1 ? rhom1s : (k == 2 ? rhom1e : rhom1c);
+    const R s = e.s[i];
+    feq[i] = std::fma(rw, std::fma(R(0.5), std::fma(s, s, e.c3), s), mw);
+  }
+}
+
+template <typename R>
+inline void equ(R rho, R vx, R vy, R vz, R* f) {
+  Eq<R> e;
+  eq_terms(rho, vx, vy, vz, e);
+  feq_from(e, f);
+  f[13] = W<R>::W0 * std::fma(e.rho, R(0.5) * e.c3, e.rhom1);
+}
+
+// calcRhoU<27>: fixed pairwise grouping, summed left to right.
+template <typename R>
+inline void moments(const R* f, R& rho, R& vx, R& vy, R& vz) {
+  rho = (R(1) + f[0]);
+  for (int k = 1; k < 27; k += 2) rho = rho + (f[k] + f[k + 1]);
+  const R inv = R(1) / rho;
+  static const int px[9] = {2, 5, 8, 11, 14, 17, 20, 23, 26};
+  static const int mx[9] = {0, 3, 6, 9, 12, 15, 18, 21, 24};
+  static const int py[9] = {6, 7, 8, 15, 16, 17, 24, 25, 26};
+  static const int my[9] = {0, 1, 2, 9, 10, 11, 18, 19, 20};
+  static const int pz[9] = {18, 19, 20, 21, 22, 23, 24, 25, 26};
+  static const int mz[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  vx = f[px[0]] - f[mx[0]];
+  vy = f[py[0]] - f[my[0]];
+  vz = f[pz[0]] - f[mz[0]];
+  for (int k = 1; k < 9; ++k) {
+    vx = vx + (f[px[k]] - f[mx[k]]);
+    vy = vy + (f[py[k]] - f[my[k]]);
+    vz = vz + (f[pz[k]] - f[mz[k]]);
+  }
+  vx = vx * inv;
+  vy = vy * inv;
+  vz = vz * inv;
+}
+
+// collision2<27>: f_i <- invTau*feq_i + (1-invTau)*f_i  (no contraction);
+// the rest direction multiplies (invTau*W0) first, as the reference does.
+template <typename R>
+inline void collide(R inv_tau, R* f) {
+  R rho, vx, vy, vz;
+  moments(f, rho, vx, vy, vz);
+  Eq<R> e;
+  eq_terms(rho, vx, vy, vz, e);
+  R feq[27];
+  feq_from(e, feq);
+  const R omt = R(1) - inv_tau;
+  for (int i = 0; i < 27; ++i) {
+    if (i == 13) continue;
+    f[i] = inv_tau * feq[i] + omt * f[i];
+  }
+  f[13] = inv_tau * W<R>::W0 * std::fma(e.rho, R(0.5) * e.c3, e.rhom1) + omt * f[13];
+}
+
+inline void bounce_back(float* f) {
+  for (int i = 0; i < 13; ++i) { const float t = f[i]; f[i] = f[26 - i]; f[26 - i] = t; }
+}
+inline void bounce_back(double* f) {
+  for (int i = 0; i < 13; ++i) { const double t = f[i]; f[i] = f[26 - i]; f[26 - i] = t; }
+}
+
+// ---- 16-bit storage codecs (round to nearest even) -------------------------
+inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+inline uint16_t f32_to_f16(float f) {
+  const uint32_t u = fbits(f);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a >= 0x7f800000u) return uint16_t(sign | 0x7c00u | (a > 0x7f800000u ? 0x200u : 0u));
+  if (a >= 0x477ff000u) return uint16_t(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (a < 0x38800000u) {                                   // result subnormal or zero
+    // value = a_float; subnormal step is 2^-24
+    const float af = bitsf(a);
+    const float q = af * 16777216.0f;  // exact scaling by 2^24
+    // round half to even to an integer
+    float r = std::nearbyint(q);
+    return uint16_t(sign | uint32_t(r));
+  }
+  const uint32_t exp = ((a >> 23) - 112u) << 10;  // rebias 127 -> 15
+  const uint32_t mant = a & 0x7fffffu;
+  uint32_t h = exp | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;
+  return uint16_t(sign | h);
+}
+
+inline float f16_to_f32(uint16_t h) {
+  const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu;
+  const uint32_t m = h & 0x3ffu;
+  if (e == 0) {
+    const float v = float(m) * (1.0f / 16777216.0f);
+    return sign ? -v : v;
+  }
+  if (e == 31) return bitsf(sign | 0x7f800000u | (m << 13));
+  return bitsf(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = fbits(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+inline float bf16_to_f32(uint16_t h) { return bitsf(uint32_t(h) << 16); }
+
+// ---- lattice helpers -------------------------------------------------------
+struct Dom {
+  int nx, ny, nz;
+  int64_t n;
+};
+
+// Pull gather with the reference's face clamp: any offset component that
+// would leave the domain is replaced by 0 (the cell pulls from itself on that
+// axis).  kernel.cu:2504-2569.
+inline int64_t pull_src(const Dom& d, int x, int y, int z, int i) {
+  const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
+  int sx = x - cx, sy = y - cy, sz = z - cz;
+  if (sx < 0 || sx >= d.nx) sx = x;
+  if (sy < 0 || sy >= d.ny) sy = y;
+  if (sz < 0 || sz >= d.nz) sz = z;
+  return (int64_t(sz) * d.ny + sy) * d.nx + sx;
+}
+
+template <typename R, typename LoadF, typename StoreF>
+void step_generic(const Dom& d, const uint32_t* flag, const float* rho_b, const float* vx_b,
+                  const float* vy_b, const float* vz_b, R inv_tau, LoadF load, StoreF store,
+                  int nthreads) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const int64_t c = (int64_t(z) * d.ny + y) * d.nx + x;
+        const uint32_t fl = flag[c];
+        R f[27];
+        for (int i = 0; i < 27; ++i) f[i] = R(0);
+        if (fl & F_LOAD)
+          for (int i = 0; i < 27; ++i) f[i] = load(i, pull_src(d, x, y, z, i));
+        if (fl & F_EQU) equ<R>(R(rho_b[c]), R(vx_b[c]), R(vy_b[c]), R(vz_b[c]), f);
+        if (fl & F_BB) bounce_back(f);
+        if (fl & F_COLLIDE) collide<R>(inv_tau, f);
+        if (fl & F_STORE)
+          for (int i = 0; i < 27; ++i) store(i, c, f[i]);
+      }
+}
+
+template <typename R, typename LoadF>
+void dump_generic(const Dom& d, const uint32_t* flag, LoadF load, float* rho_b, float* vx_b,
+                  float* vy_b, float* vz_b, int nthreads) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const int64_t c = (int64_t(z) * d.ny + y) * d.nx + x;
+        const uint32_t fl = flag[c];
+        R f[27];
+        for (int i = 0; i < 27; ++i) f[i] = load(i, pull_src(d, x, y, z, i));
+        R rho, vx, vy, vz;
+        moments(f, rho, vx, vy, vz);
+        if (fl & F_DUMP_RHO) rho_b[c] = float(rho);
+        if (fl & F_DUMP_VX) vx_b[c] = float(vx);
+        if (fl & F_DUMP_VY) vy_b[c] = float(vy);
+        if (fl & F_DUMP_VZ) vz_b[c] = float(vz);
+      }
+}
+
+// FP64 dump keeps double moments (the accuracy oracle keeps its own buffers).
+void dump_f64_impl(const Dom& d, const uint32_t* flag, const double* src, double* rho_b,
+                   double* vx_b, double* vy_b, double* vz_b, int nthreads) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const int64_t c = (int64_t(z) * d.ny + y) * d.nx + x;
+        const uint32_t fl = flag[c];
+        double f[27];
+        for (int i = 0; i < 27; ++i) f[i] = src[int64_t(i) * d.n + pull_src(d, x, y, z, i)];
+        double rho, vx, vy, vz;
+        moments(f, rho, vx, vy, vz);
+        if (fl & F_DUMP_RHO) rho_b[c] = rho;
+        if (fl & F_DUMP_VX) vx_b[c] = vx;
+        if (fl & F_DUMP_VY) vy_b[c] = vy;
+        if (fl & F_DUMP_VZ) vz_b[c] = vz;
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- velocity set (velocity_set.hpp:189-255) -------------------------------
+void orc_velset(int32_t* cx, int32_t* cy, int32_t* cz, double* w) {
+  for (int i = 0; i < 27; ++i) {
+    cx[i] = cx_of(i);
+    cy[i] = cy_of(i);
+    cz[i] = cz_of(i);
+    const int k = wclass(i);
+    w[i] = k == 0 ? 8.0 / 27 : (k == 1 ? 2.0 / 27 : (k == 2 ? 1.0 / 54 : 1.0 / 216));
+  }
+}
+
+// ---- per-cell functions ----------------------------------------------------
+void orc_equ_f64(double rho, double vx, double vy, double vz, double* f) { equ<double>(rho, vx, vy, vz, f); }
+void orc_equ_f32(float rho, float vx, float vy, float vz, float* f) { equ<float>(rho, vx, vy, vz, f); }
+
+void orc_moments_f64(const double* f, double* out4) { moments<double>(f, out4[0], out4[1], out4[2], out4[3]); }
+void orc_moments_f32(const float* f, float* out4) { moments<float>(f, out4[0], out4[1], out4[2], out4[3]); }
+
+void orc_collide_f64(double inv_tau, double* f) { collide<double>(inv_tau, f); }
+void orc_collide_f32(float inv_tau, float* f) { collide<float>(inv_tau, f); }
+
+// Batched per-cell collide: f is [ncell][27].
+void orc_collide_batch_f32(float inv_tau, float* f, int64_t ncell) {
+  for (int64_t c = 0; c < ncell; ++c) collide<float>(inv_tau, f + 27 * c);
+}
+void orc_equ_batch_f32(const float* rho, const float* vx, const float* vy, const float* vz, float* f,
+                       int64_t ncell) {
+  for (int64_t c = 0; c < ncell; ++c) equ<float>(rho[c], vx[c], vy[c], vz[c], f + 27 * c);
+}
+void orc_moments_batch_f32(const float* f, float* out, int64_t ncell) {
+  for (int64_t c = 0; c < ncell; ++c)
+    moments<float>(f + 27 * c, out[4 * c], out[4 * c + 1], out[4 * c + 2], out[4 * c + 3]);
+}
+
+// Equilibrium of rho/u fields as SoA populations f[27][n] (initial state).
+void orc_feq_soa_f64(const float* rho, const float* vx, const float* vy, const float* vz, double* f,
+                     int64_t n) {
+  for (int64_t c = 0; c < n; ++c) {
+    double g[27];
+    equ<double>(rho[c], vx[c], vy[c], vz[c], g);
+    for (int i = 0; i < 27; ++i) f[int64_t(i) * n + c] = g[i];
+  }
+}
+void orc_feq_soa_f32(const float* rho, const float* vx, const float* vy, const float* vz, float* f,
+                     int64_t n) {
+  for (int64_t c = 0; c < n; ++c) {
+    float g[27];
+    equ<float>(rho[c], vx[c], vy[c], vz[c], g);
+    for (int i = 0; i < 27; ++i) f[int64_t(i) * n + c] = g[i];
+  }
+}
+
+// ---- lattice steps (SoA [27][nz][ny][nx], x fastest) ------------------------
+// One plain pull step, double buffered (kernel.cu:2489-2632).  Cells without
+// STORE leave dst untouched.
+void orc_step_f64(int nx, int ny, int nz, const uint32_t* flag, const float* rho, const float* vx,
+                  const float* vy, const float* vz, double inv_tau, const double* src, double* dst,
+                  int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  step_generic<double>(
+      d, flag, rho, vx, vy, vz, inv_tau,
+      [&](int i, int64_t s) { return src[int64_t(i) * d.n + s]; },
+      [&](int i, int64_t c, double v) { dst[int64_t(i) * d.n + c] = v; }, nthreads);
+}
+
+void orc_step_f32(int nx, int ny, int nz, const uint32_t* flag, const float* rho, const float* vx,
+                  const float* vy, const float* vz, float inv_tau, const float* src, float* dst,
+                  int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  step_generic<float>(
+      d, flag, rho, vx, vy, vz, inv_tau,
+      [&](int i, int64_t s) { return src[int64_t(i) * d.n + s]; },
+      [&](int i, int64_t c, float v) { dst[int64_t(i) * d.n + c] = v; }, nthreads);
+}
+
+// 16-bit shifted storage: stored h = encode(scale * f); decoded f = decode(h) * (1/scale).
+// fmt 1 = IEEE binary16, fmt 2 = bfloat16.  Collision in FP32.
+void orc_step_s16(int nx, int ny, int nz, const uint32_t* flag, const float* rho, const float* vx,
+                  const float* vy, const float* vz, float inv_tau, int fmt, float scale,
+                  const uint16_t* src, uint16_t* dst, int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  const float inv_scale = 1.0f / scale;
+  step_generic<float>(
+      d, flag, rho, vx, vy, vz, inv_tau,
+      [&](int i, int64_t s) {
+        const uint16_t h = src[int64_t(i) * d.n + s];
+        return (fmt == 1 ? f16_to_f32(h) : bf16_to_f32(h)) * inv_scale;
+      },
+      [&](int i, int64_t c, float v) {
+        const float t = v * scale;
+        dst[int64_t(i) * d.n + c] = fmt == 1 ? f32_to_f16(t + 0.0f) : f32_to_bf16(t);
+      },
+      nthreads);
+}
+
+// Dump: rho/u from the pull-gathered populations of src, written only where
+// the DUMP bits are set (kernel.cu:2634-2735).
+void orc_dump_f32(int nx, int ny, int nz, const uint32_t* flag, const float* src, float* rho,
+                  float* vx, float* vy, float* vz, int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  dump_generic<float>(
+      d, flag, [&](int i, int64_t s) { return src[int64_t(i) * d.n + s]; }, rho, vx, vy, vz,
+      nthreads);
+}
+
+void orc_dump_s16(int nx, int ny, int nz, const uint32_t* flag, int fmt, float scale,
+                  const uint16_t* src, float* rho, float* vx, float* vy, float* vz, int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  const float inv_scale = 1.0f / scale;
+  dump_generic<float>(
+      d, flag,
+      [&](int i, int64_t s) {
+        const uint16_t h = src[int64_t(i) * d.n + s];
+        return (fmt == 1 ? f16_to_f32(h) : bf16_to_f32(h)) * inv_scale;
+      },
+      rho, vx, vy, vz, nthreads);
+}
+
+void orc_dump_f64(int nx, int ny, int nz, const uint32_t* flag, const double* src, double* rho,
+                  double* vx, double* vy, double* vz, int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  dump_f64_impl(d, flag, src, rho, vx, vy, vz, nthreads);
+}
+
+// ---- codecs (vector forms, for tests) --------------------------------------
+// Storage encode of already-scaled values; binary16 stores -0 as +0.
+void orc_encode16(int fmt, const float* in, uint16_t* out, int64_t n) {
+  for (int64_t k = 0; k < n; ++k) out[k] = fmt == 1 ? f32_to_f16(in[k] + 0.0f) : f32_to_bf16(in[k]);
+}
+void orc_decode16(int fmt, const uint16_t* in, float* out, int64_t n) {
+  for (int64_t k = 0; k < n; ++k) out[k] = fmt == 1 ? f16_to_f32(in[k]) : bf16_to_f32(in[k]);
+}
+
+// ---- halo-blocking axis math (blocking_alogrithm.hpp:8-41) -----------------
+// Restated exactly, including the reference's validity test.
+int orc_blk_valid(int dom, int blk, int iter) {
+  if (blk < dom && blk + 1 <= iter) return 0;
+  if (2 * (blk + 1 - iter) < dom && blk + 2 <= iter) return 0;
+  return 1;
+}
+
+int orc_blk_num(int dom, int blk, int iter) {
+  if (dom <= blk) return 1;
+  if (dom <= 2 * (blk + 1 - iter)) return 2;
+  const unsigned num = unsigned(dom - 2 * (blk + 1 - iter));
+  const unsigned den = unsigned(blk + 2 - 2 * iter);
+  return 2 + int((num + den - 1) / den);
+}
+
+int orc_valid_prev(int idx, int blk, int num, int iter, int dom) {
+  if (idx == num) return dom;
+  if (idx == 0) return 0;
+  return (blk + 1 - iter) + (idx - 1) * (blk + 2 - 2 * iter);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

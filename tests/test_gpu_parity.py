@@ -1,0 +1,263 @@
+"""GPU parity tests: the HIP path through the C ABI against the oracle.
+
+Bars:
+  * FP32 plain pull == FP32 CPU restatement, bit for bit (same operation order);
+  * FP32 rho/u vs the FP64 oracle: max |drho| <= 1e-5 (relative to rho ~ 1),
+    max |du| <= 1e-5 * u_in;
+  * every in-place / blocked variant == plain pull of the same storage, bit for bit;
+  * 16-bit storage == CPU 16-bit restatement, bit for bit;
+  * per-cell physics == the reference's own functions (golden vectors), bit for bit.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+from culbm_amd import Lattice, LatticeConfig, cases
+from culbm_amd import lattice as L
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32 if a.dtype == np.float32 else np.uint16)
+
+
+def assert_bitexact(a, b, what=""):
+    assert a.shape == b.shape, what
+    ba, bb = bits(a), bits(b)
+    bad = np.flatnonzero(ba.reshape(-1) != bb.reshape(-1))
+    assert bad.size == 0, f"{what}: {bad.size} values differ, first at flat index {bad[:5]}"
+
+
+def make(state, init="rest", **cfg):
+    lat = Lattice(LatticeConfig(domain=state.domain, **cfg))
+    lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
+    lat.init_ddf(init)
+    return lat
+
+
+# ---- per-cell physics vs the reference's functions --------------------------
+
+def test_cell_physics_matches_reference_golden(golden_dir):
+    g = np.load(golden_dir / "cell_physics.npz")
+    for k, it in enumerate(g["inv_taus"]):
+        out = L.probe_collide(g["f_in"], float(it))
+        assert_bitexact(out, g["collide_out"][k], f"collision2<27> invTau={it}")
+    assert_bitexact(L.probe_equilibrium(g["q_in"]), g["equ_out"], "calcEqu<27>")
+    assert_bitexact(L.probe_moments(g["f_in"]), g["moments_out"], "calcRhoU<27>")
+
+
+def test_codec_matches_host_restatement():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        rng.normal(0, 1e-2, 20000), rng.normal(0, 1e-6, 5000), rng.uniform(-20, 20, 5000),
+        np.array([0.0, -0.0, 1.0, -1.0, 65504.0, 65520.0, 1e-8, 6.0e-8, 2.0 ** -24 * 1.5,
+                  2.0 ** -14, 2.0 ** -14 * (1 - 2.0 ** -11)])
+    ]).astype(np.float32)
+    for storage, fmt, scale in (("fp16", 1, 1.0), ("fp16", 1, 4096.0), ("bf16", 2, 1.0), ("bf16", 2, 256.0)):
+        enc, dec = L.probe_codec(x, storage, scale)
+        want = np.zeros(x.shape, np.uint16)
+        scaled = x * np.float32(scale)
+        pyoracle.lib().orc_encode16(fmt, scaled, want, x.size)
+        assert np.array_equal(enc, want), f"{storage} encode (scale {scale})"
+        back = np.zeros(x.shape, np.float32)
+        pyoracle.lib().orc_decode16(fmt, want, back, x.size)
+        assert_bitexact(dec, back * np.float32(1.0 / scale), f"{storage} decode")
+
+
+# ---- plain pull vs the oracle -------------------------------------------------
+
+def test_plain_pull_small_config_vs_oracle():
+    """64^3 channel, zero-gradient outlet, rho = 1 + U(-1e-3, 1e-3), 100 steps."""
+    st = cases.config_small(seed=1)
+    inv_tau = 0.5
+    lat = make(st, init="feq", inv_tau=inv_tau)
+    lat.step(100)
+    lat.dump()
+    g_rho, g_vx, g_vy, g_vz = lat.macros()
+    g_ddf = lat.get_ddf()
+
+    o32 = pyoracle.Sim(st, inv_tau, np.float32)
+    o32.init_feq()
+    o32.step(100)
+    r32 = o32.dump()
+    assert_bitexact(g_ddf, o32.ddf(), "FP32 populations after 100 steps")
+    for a, b, name in zip((g_rho, g_vx, g_vy, g_vz), r32, ("rho", "vx", "vy", "vz")):
+        assert_bitexact(a, b, f"FP32 {name}")
+
+    o64 = pyoracle.Sim(st, inv_tau, np.float64)
+    o64.init_feq()
+    o64.step(100)
+    rho64, vx64, vy64, vz64 = o64.dump()
+    fluid = (st.flag & cases.DUMP) != 0
+    drho = np.abs(g_rho.astype(np.float64) - rho64)[fluid].max()
+    du = max(np.abs(a.astype(np.float64) - b)[fluid].max() for a, b in ((g_vx, vx64), (g_vy, vy64), (g_vz, vz64)))
+    assert drho <= 1e-5, drho
+    assert du <= 1e-5 * 0.05, du
+    # the flow really developed (not a trivially resting state)
+    assert np.abs(vx64[fluid]).max() > 1e-3
+
+
+@pytest.mark.parametrize("storage,fmt", [("fp16", "fp16"), ("bf16", "bf16")])
+def test_plain_pull_16bit_vs_oracle(storage, fmt):
+    st = cases.config_small(seed=7, n=40)
+    lat = make(st, init="feq", inv_tau=0.6, storage=storage)
+    scale = lat.info()["storage_scale"]
+    lat.step(30)
+    lat.dump()
+    g = lat.macros()
+    o = pyoracle.Sim(st, 0.6, fmt, scale=scale)
+    o.init_feq()
+    o.step(30)
+    r = o.dump()
+    assert_bitexact(lat.get_ddf(), o.ddf(), f"{storage} populations")
+    for a, b in zip(g, r):
+        assert_bitexact(a, b, f"{storage} macros")
+
+
+# ---- streaming KAT: index tags through pure streaming ------------------------
+
+def stream_expect(tags, steps):
+    """Pure streaming with the reference's face clamp, repeated `steps` times."""
+    cx, cy, cz, _ = pyoracle.velset()
+    _, nz, ny, nx = tags.shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    cur = tags.copy()
+    for _ in range(steps):
+        nxt = np.empty_like(cur)
+        for i in range(27):
+            sx, sy, sz = x - cx[i], y - cy[i], z - cz[i]
+            sx = np.where((sx < 0) | (sx >= nx), x, sx)
+            sy = np.where((sy < 0) | (sy >= ny), y, sy)
+            sz = np.where((sz < 0) | (sz >= nz), z, sz)
+            nxt[i] = cur[i][sz, sy, sx]
+        cur = nxt
+    return cur
+
+
+POLICIES = [(0, 0), (1, 0), (0, 3), (1, 3), (0, 1), (1, 1)]
+
+
+@pytest.mark.parametrize("sp,op", POLICIES)
+def test_stream_index_tags(sp, op):
+    nx, ny, nz = 13, 9, 11
+    st = cases.channel(nx, ny, nz)
+    st.flag[...] = cases.LOAD | cases.STORE
+    lat = Lattice(LatticeConfig(domain=(nx, ny, nz), stream_policy=sp, opt_policy=op, inner_loop=3,
+                                wave_depth=2))
+    lat.set_state(st.flag)
+    n = nx * ny * nz
+    tags = (np.arange(27)[:, None] * n + np.arange(n)[None, :]).astype(np.float32).reshape(27, nz, ny, nx)
+    lat.set_ddf(tags)
+    for steps in (1, 4):
+        lat.set_ddf(tags)
+        lat.step(steps)
+        got = lat.get_ddf()
+        want = stream_expect(tags, steps)
+        assert np.array_equal(got, want), f"policy ({sp},{op}) after {steps} steps"
+
+
+# ---- every variant equals plain pull bit for bit -----------------------------
+
+def obstacle_case(nx=37, ny=29, nz=31, outlet="zero_gradient", seed=11):
+    st = cases.channel(nx, ny, nz, u_in=0.06, outlet=outlet)
+    cases.add_spheres(st, 6, 2.5, 5.0, seed)
+    rng = np.random.default_rng(seed)
+    st.rho[...] = (1.0 + rng.uniform(-2e-3, 2e-3, st.shape)).astype(np.float32)
+    return st
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("outlet", ["zero_gradient", "equilibrium"])
+def test_variants_bitexact_vs_plain(storage, outlet):
+    st = obstacle_case(outlet=outlet)
+    steps = 17
+    ref = make(st, init="feq", inv_tau=0.7, storage=storage)
+    ref.step(steps)
+    ref.dump()
+    want_m = ref.macros()
+    want_f = ref.get_ddf()
+    for sp, op, il, wd in [(1, 0, 5, 0), (0, 3, 5, 3), (1, 3, 5, 3), (1, 3, 1, 2), (1, 3, 4, 0),
+                           (0, 1, 3, 4), (1, 1, 5, 2), (1, 1, 2, 0)]:
+        lat = make(st, init="feq", inv_tau=0.7, storage=storage, stream_policy=sp, opt_policy=op,
+                   inner_loop=il, wave_depth=wd)
+        lat.step(steps)
+        lat.dump()
+        what = f"({sp},{op}) I={il} D={wd} {storage} {outlet}"
+        assert_bitexact(lat.get_ddf(), want_f, what + " populations")
+        for a, b in zip(lat.macros(), want_m):
+            assert_bitexact(a, b, what + " macros")
+        lat.close()
+
+
+def test_window_semantics():
+    st = obstacle_case()
+    lat = make(st, stream_policy=1, opt_policy=3, inner_loop=4)
+    steps, ms = lat.run_window(10)
+    assert steps == 12 and ms > 0
+    assert lat.info()["step"] == 12
+    plain = make(st)
+    steps, _ = plain.run_window(10)
+    assert steps == 10
+
+
+# ---- dump semantics ------------------------------------------------------------
+
+def test_dump_writes_only_flagged_cells():
+    st = obstacle_case()
+    st.flag[:, :, : st.shape[2] // 2] &= ~np.uint32(cases.DUMP_VY)
+    st.vy[...] = 7.0  # sentinel in the EQU/dump buffer
+    st.vy[:, :, 0] = 0.0
+    lat = make(st, inv_tau=0.6)
+    lat.step(5)
+    lat.dump()
+    rho, vx, vy, vz = lat.macros()
+    no_dump = (st.flag & cases.DUMP_VY) == 0
+    assert np.all(vy[no_dump] == st.vy[no_dump])
+    assert np.all(vy[~no_dump] != 7.0)
+    o = pyoracle.Sim(st, 0.6, np.float32)
+    o.step(5)
+    r = o.dump()
+    for a, b in zip((rho, vx, vy, vz), r):
+        assert_bitexact(a, b, "dump")
+
+
+@pytest.mark.parametrize("sp,op", [(0, 0), (1, 0), (1, 3)])
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_set_get_ddf_roundtrip(sp, op, storage):
+    st = obstacle_case(nx=19, ny=17, nz=15)
+    lat = make(st, stream_policy=sp, opt_policy=op, storage=storage)
+    rng = np.random.default_rng(3)
+    f = rng.normal(0, 1e-2, (27,) + st.shape).astype(np.float32)
+    if storage == "fp16":
+        scale = lat.info()["storage_scale"]
+        enc = np.zeros(f.size, np.uint16)
+        pyoracle.lib().orc_encode16(1, (f * np.float32(scale)).reshape(-1), enc, f.size)
+        dec = np.zeros(f.size, np.float32)
+        pyoracle.lib().orc_decode16(1, enc, dec, f.size)
+        f = (dec * np.float32(1 / scale)).reshape(f.shape)
+    lat.set_ddf(f)
+    assert_bitexact(lat.get_ddf(), f, "roundtrip")
+
+
+# ---- the optimised workload at full size --------------------------------------
+
+@pytest.mark.slow
+def test_full_size_blocked_equals_plain():
+    """288x272x280, 60 spheres: in-place + wavefront blocking (I = 5) vs plain pull."""
+    st = cases.config_obstacles()
+    steps = 25
+    out = []
+    for sp, op in ((0, 0), (1, 3)):
+        lat = make(st, inv_tau=0.5, stream_policy=sp, opt_policy=op, inner_loop=5)
+        lat.step(steps)
+        lat.dump()
+        out.append((lat.macros(), lat.get_ddf()))
+        lat.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert_bitexact(a, b, "full-size macros")
+    assert_bitexact(out[0][1], out[1][1], "full-size populations")
