@@ -146,6 +146,15 @@ def main(argv=None) -> int:
         cpu = cpu_baseline(state)
 
     achieved = head["kernel_gbs"]
+    # HBM bytes per launch of the dominant kernel: the counter-derived bytes per
+    # cell-update committed under profiles/ times the cells one launch updates
+    traffic = None
+    tpath = ROOT / "profiles" / "traffic_288x272x280.json"
+    if tpath.exists():
+        t = json.loads(tpath.read_text()).get(ns.form)
+        if t:
+            cells_per_launch = DOMAIN[0] * DOMAIN[1] * DOMAIN[2] / head["launches_per_step"]
+            traffic = round(t["bytes_per_cell"] * cells_per_launch)
     line = {
         "metric": "MLUPS D3Q27 at 288x272x280 (% HBM roofline); adjoint/forward step ratio",
         "value": round(head["mlups"], 3),
@@ -175,7 +184,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel_avg_us": round(head["kernel_avg_us"], 2),
             "launches_per_step": head["launches_per_step"],
             "bytes_per_update": head["bytes_per_update"],

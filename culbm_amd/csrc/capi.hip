@@ -38,7 +38,9 @@ struct culbm_ctx {
   int64_t step = 0;
   float scale = 1.0f, inv_scale = 1.0f;
   int32_t wave_depth = 0;
-  int32_t coop_blocks = 0;
+  // static policy: one instantiated graph per starting state of a full pass
+  // (pull: index of the current buffer; in-place: parity of the next step)
+  hipGraphExec_t pass_graph[2] = {nullptr, nullptr};
   int64_t device_bytes = 0;
   // per-launch timing (culbm_profile_steps): an event pair around every launch
   bool prof = false;
@@ -275,26 +277,39 @@ int wave_pass(culbm_ctx* ctx, int32_t T) {
   return CULBM_OK;
 }
 
+// Static blocking: the wavefront schedule of a full pass is fixed by the
+// configuration, so it is captured once into a HIP graph and replayed; the
+// device's command processor then issues the chunk steps back to back with
+// no host round trip.  A shorter final pass uses plain launches.
 template <class S>
-int coop_pass(culbm_ctx* ctx, int32_t T) {
-  WaveArgs w{};
-  w.a = base_args(ctx);
-  w.depth = ctx->wave_depth;
-  w.nsteps = T;
-  w.parity0 = ctx->parity;
-  if (ctx->inplace) {
-    w.a.src = ctx->buf[0];
-    w.a.dst = ctx->buf[0];
-  } else {
-    w.buf0 = ctx->buf[ctx->cur];
-    w.buf1 = ctx->buf[ctx->cur ^ 1];
+int static_pass(culbm_ctx* ctx, int32_t T) {
+  if (T != ctx->cfg.inner_loop) return wave_pass<S>(ctx, T);
+  const int key = ctx->inplace ? ctx->parity : ctx->cur;
+  if (!ctx->pass_graph[key]) {
+    const int cur = ctx->cur, parity = ctx->parity;
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = wave_pass<S>(ctx, T);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->prof = prof;
+    ctx->cur = cur;
+    ctx->parity = parity;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, CULBM_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&ctx->pass_graph[key], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      ctx->pass_graph[key] = nullptr;
+      return fail(ctx, CULBM_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
   }
-  void* args[] = {&w};
   if (int rc = prof_begin(ctx)) return rc;
-  const void* fn = ctx->inplace ? reinterpret_cast<const void*>(&k_wave_coop<S, true>)
-                                : reinterpret_cast<const void*>(&k_wave_coop<S, false>);
-  HIPCHK(ctx, hipLaunchCooperativeKernel(fn, dim3(ctx->coop_blocks), dim3(kBlock), args, 0,
-                                         ctx->stream));
+  HIPCHK(ctx, hipGraphLaunch(ctx->pass_graph[key], ctx->stream));
   if (int rc = prof_end(ctx)) return rc;
   ctx->cur = (ctx->cur + T) & 1;
   ctx->parity = (ctx->parity + T) & 1;
@@ -320,7 +335,7 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
     int32_t left = nsteps;
     while (left > 0) {
       const int32_t T = std::min(left, c.inner_loop);
-      const int rc = c.opt_policy == CULBM_OPT_STATIC ? coop_pass<S>(ctx, T) : wave_pass<S>(ctx, T);
+      const int rc = c.opt_policy == CULBM_OPT_STATIC ? static_pass<S>(ctx, T) : wave_pass<S>(ctx, T);
       if (rc) return rc;
       left -= T;
     }
@@ -516,24 +531,6 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
 
   if (blocked(c)) {
     ctx->wave_depth = c.wave_depth > 0 ? std::min(c.wave_depth, c.nz) : auto_depth(ctx);
-    if (c.opt_policy == CULBM_OPT_STATIC) {
-      int per_cu = 0;
-      const void* fn = nullptr;
-      by_storage(c.storage, [&](auto s) {
-        using S = decltype(s);
-        fn = ctx->inplace ? reinterpret_cast<const void*>(&k_wave_coop<S, true>)
-                          : reinterpret_cast<const void*>(&k_wave_coop<S, false>);
-        return 0;
-      });
-      HK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
-      hipDeviceProp_t prop;
-      HK(hipGetDeviceProperties(&prop, c.device));
-      if (per_cu < 1) {
-        fail(ctx, CULBM_ERR_UNSUPPORTED, "cooperative wavefront kernel cannot be resident");
-        return bail(CULBM_ERR_UNSUPPORTED);
-      }
-      ctx->coop_blocks = per_cu * prop.multiProcessorCount;
-    }
   }
 
   // reference defaults: all FLUID + DUMP, rho 1, u 0, rest populations
@@ -556,6 +553,8 @@ int culbm_destroy(culbm_ctx* ctx) {
   void* ptrs[] = {ctx->buf[0], ctx->buf[1], ctx->flag, ctx->sflag, ctx->rho, ctx->vx, ctx->vy, ctx->vz};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipGraphExec_t g : ctx->pass_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -750,8 +749,6 @@ int culbm_get_info(culbm_ctx* ctx, culbm_info* info) {
   info->storage_scale = ctx->scale;
   const culbm_config& c = ctx->cfg;
   if (!blocked(c)) {
-    info->launches_per_pass = 1;
-  } else if (c.opt_policy == CULBM_OPT_STATIC) {
     info->launches_per_pass = 1;
   } else {
     const int32_t T = c.inner_loop, D = ctx->wave_depth, nz = c.nz;

@@ -18,7 +18,6 @@
 //    that consumer's load slot ("ghost" store).  Ghost slots are never in
 //    any cell's current load set (their regular writer would lie outside the
 //    domain), so ghost stores cannot race with loads of the same step.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "lbm_kernels.hpp"
@@ -59,25 +58,37 @@ __device__ __forceinline__ Cell locate(const StepArgs& a, uint32_t blk) {
 }
 
 // ---------------------------------------------------------------------------
+// memory access: a wave-uniform base address (SGPRs) plus one 32-bit per-lane
+// byte offset, the form gfx950 issues as global_load/store with `saddr`, so a
+// direction costs no 64-bit address arithmetic.  Direction planes are
+// `dsb` bytes apart; every per-lane offset stays below 2^32 (checked on the host).
+
+template <class T>
+__device__ __forceinline__ T ld_at(const char* base, uint32_t off) {
+  return *reinterpret_cast<const T*>(base + off);
+}
+
+template <class T>
+__device__ __forceinline__ void st_at(char* base, uint32_t off, T v) {
+  *reinterpret_cast<T*>(base + off) = v;
+}
+
+// ---------------------------------------------------------------------------
 // pull layout
 
 template <class S>
 struct PullGather {
-  const typename S::T* base;
-  int64_t ds;
-  int32_t xm, xp, ym, yp, zm, zp;  // clamped offsets to x-1, x+1, ...
-  uint32_t l;
+  const char* base;
+  int64_t dsb;
+  uint32_t o[3][3];  // byte offset of the clamped source row (y - cy, z - cz), [cy+1][cz+1]
+  int32_t xm, xp;    // byte offset to x-1 / x+1 (0 when clamped on that face)
   float inv_s;
 
   template <int I>
   __device__ __forceinline__ void run(float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
-    // source cell x - c_i
-    const int32_t ox = cx == 1 ? xm : (cx == -1 ? xp : 0);
-    const int32_t oy = cy == 1 ? ym : (cy == -1 ? yp : 0);
-    const int32_t oz = cz == 1 ? zm : (cz == -1 ? zp : 0);
-    const typename S::T* p = base + I * ds;
-    f[I] = S::dec(p[uint32_t(int32_t(l) + ox + oy + oz)], inv_s);
+    const int32_t ox = cx == 1 ? xm : (cx == -1 ? xp : 0);  // source x - c_x
+    f[I] = S::dec(ld_at<typename S::T>(base + I * dsb, o[cy + 1][cz + 1] + uint32_t(ox)), inv_s);
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
@@ -85,26 +96,43 @@ struct PullGather {
 template <class S>
 __device__ __forceinline__ PullGather<S> make_gather(const StepArgs& a, const Cell& k,
                                                      const void* src) {
+  constexpr int32_t B = sizeof(typename S::T);
   PullGather<S> g;
-  g.base = static_cast<const typename S::T*>(src);
-  g.ds = a.dstride;
-  const int32_t plane = a.px * a.py;
-  g.xm = k.x == 0 ? 0 : -1;
-  g.xp = k.x == a.nx - 1 ? 0 : 1;
-  g.ym = k.y == 0 ? 0 : -a.px;
-  g.yp = k.y == a.ny - 1 ? 0 : a.px;
-  g.zm = k.z == 0 ? 0 : -plane;
-  g.zp = k.z == a.nz - 1 ? 0 : plane;
-  g.l = k.l;
+  g.base = static_cast<const char*>(src);
+  g.dsb = a.dstride * B;
+  const int32_t row = a.px * B, plane = a.px * a.py * B;
+  const uint32_t lb = k.l * uint32_t(B);
+  // source offsets for c = -1, 0, +1 (the source sits at -c), clamped at faces
+  const int32_t oy[3] = {k.y == a.ny - 1 ? 0 : row, 0, k.y == 0 ? 0 : -row};
+  const int32_t oz[3] = {k.z == a.nz - 1 ? 0 : plane, 0, k.z == 0 ? 0 : -plane};
+#pragma unroll
+  for (int jy = 0; jy < 3; ++jy)
+#pragma unroll
+    for (int jz = 0; jz < 3; ++jz) g.o[jy][jz] = lb + uint32_t(oy[jy] + oz[jz]);
+  g.xm = k.x == 0 ? 0 : -B;
+  g.xp = k.x == a.nx - 1 ? 0 : B;
   g.inv_s = a.inv_scale;
   return g;
 }
 
 template <class S, int I>
-__device__ __forceinline__ void store_plain(typename S::T* base, int64_t ds, uint32_t l,
-                                            const float* f, float s) {
-  base[I * ds + l] = S::enc(f[I], s);
-  if constexpr (I < 26) store_plain<S, I + 1>(base, ds, l, f, s);
+__device__ __forceinline__ void store_plain(char* base, int64_t dsb, uint32_t lb, const float* f,
+                                            float s) {
+  st_at<typename S::T>(base + I * dsb, lb, S::enc(f[I], s));
+  if constexpr (I < 26) store_plain<S, I + 1>(base, dsb, lb, f, s);
+}
+
+template <class S>
+__device__ __forceinline__ void store_plain_all(const StepArgs& a, const Cell& k, void* dst,
+                                                const float* f) {
+  constexpr uint32_t B = sizeof(typename S::T);
+  store_plain<S, 0>(static_cast<char*>(dst), a.dstride * B, k.l * B, f, a.scale);
+}
+
+// Cells without LOAD start from zero populations.
+__device__ __forceinline__ void clear(float* f) {
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = 0.0f;
 }
 
 // The per-cell update shared by every variant: LOAD -> EQU -> BB -> COLLIDE
@@ -119,197 +147,154 @@ template <class S>
 __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
   if (!k.ok) return;
-  const uint32_t fl = a.sflag[k.c];
+  // The gather is in bounds for every cell (clamped), so it is issued before
+  // the flag arrives instead of waiting on it.
   float f[27];
-#pragma unroll
-  for (int i = 0; i < 27; ++i) f[i] = 0.0f;
-  if (fl & kLoad) make_gather<S>(a, k, a.src).template run<0>(f);
+  make_gather<S>(a, k, a.src).template run<0>(f);
+  const uint32_t fl = a.sflag[k.c];
+  if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, f);
-  if (fl & kStore)
-    store_plain<S, 0>(static_cast<typename S::T*>(a.dst), a.dstride, k.l, f, a.scale);
+  if (fl & kStore) store_plain_all<S>(a, k, a.dst, f);
 }
 
 // ---------------------------------------------------------------------------
 // in-place (twisted) layout
 
+// Byte offsets of the four row/plane combinations {x, x+row, x+plane, x+row+plane}.
+struct TwistRows {
+  uint32_t o[2][2];
+  uint32_t lb;
+  int32_t row, plane;
+};
+
+template <int B>
+__device__ __forceinline__ TwistRows twist_rows(const StepArgs& a, const Cell& k) {
+  TwistRows r;
+  r.row = a.px * B;
+  r.plane = a.px * a.py * B;
+  r.lb = k.l * uint32_t(B);
+  r.o[0][0] = r.lb;
+  r.o[1][0] = r.lb + uint32_t(r.row);
+  r.o[0][1] = r.lb + uint32_t(r.plane);
+  r.o[1][1] = r.lb + uint32_t(r.row + r.plane);
+  return r;
+}
+
 // Loads of a step of parity P (1 = O: slot i, 0 = E: slot 26-i) at x + p(-c_i).
 template <class S, int P>
 struct TwistLoad {
-  const typename S::T* base;
-  int64_t ds;
-  int32_t sx, sy, sz;  // layout strides of one step along x, y, z
-  uint32_t l;
+  const char* base;
+  int64_t dsb;
+  TwistRows r;
   float inv_s;
   template <int I>
   __device__ __forceinline__ void run(float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? I : 26 - I;
-    const int32_t o = (cx == -1 ? sx : 0) + (cy == -1 ? sy : 0) + (cz == -1 ? sz : 0);
-    const typename S::T* p = base + slot * ds;
-    f[I] = S::dec(p[l + uint32_t(o)], inv_s);
+    constexpr uint32_t ox = cx == -1 ? sizeof(typename S::T) : 0;
+    f[I] = S::dec(ld_at<typename S::T>(base + slot * dsb, r.o[cy == -1][cz == -1] + ox), inv_s);
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
 
-// Stores of a step of parity P (1 = O: slot 26-i, 0 = E: slot i) at
-// x + p(c_i), plus the ghost copies for consumers clamped onto this cell.
+// Stores of a step of parity P (1 = O: slot 26-i, 0 = E: slot i) at x + p(c_i).
 template <class S, int P>
 struct TwistStore {
-  typename S::T* base;
-  int64_t ds;
-  int32_t sx, sy, sz;
-  uint32_t l;
+  char* base;
+  int64_t dsb;
+  TwistRows r;
   float s;
-  // face[a] bit 0: coordinate is 0 on axis a; bit 1: coordinate is n_a - 1
-  uint32_t fx, fy, fz;
-  bool face;
-
   template <int I>
   __device__ __forceinline__ void run(const float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? 26 - I : I;
-    const typename S::T v = S::enc(f[I], s);
-    typename S::T* p = base + slot * ds;
-    const int32_t ox = cx == 1 ? sx : 0, oy = cy == 1 ? sy : 0, oz = cz == 1 ? sz : 0;
-    p[l + uint32_t(ox + oy + oz)] = v;
-    if (face) {
-      // a consumer is clamped on axis a when x_a - c_a leaves the domain:
-      // c_a = +1 on the low face, c_a = -1 on the high face.
-      const uint32_t gx = cx == 1 ? (fx & 1u) : (cx == -1 ? (fx >> 1) : 0u);
-      const uint32_t gy = cy == 1 ? (fy & 1u) : (cy == -1 ? (fy >> 1) : 0u);
-      const uint32_t gz = cz == 1 ? (fz & 1u) : (cz == -1 ? (fz >> 1) : 0u);
-      const uint32_t gm = gx | (gy << 1) | (gz << 2);
-      if (gm) {
-        // ghost offset on a clamped axis is p(-c_a)
-        const int32_t qx = cx == -1 ? sx : 0, qy = cy == -1 ? sy : 0, qz = cz == -1 ? sz : 0;
-#pragma unroll
-        for (uint32_t m = 1; m < 8; ++m) {
-          if ((m & ~gm) == 0) {
-            const int32_t o = ((m & 1u) ? qx : ox) + ((m & 2u) ? qy : oy) + ((m & 4u) ? qz : oz);
-            p[l + uint32_t(o)] = v;
-          }
-        }
-      }
+    constexpr uint32_t ox = cx == 1 ? sizeof(typename S::T) : 0;
+    st_at<typename S::T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, S::enc(f[I], s));
+    if constexpr (I < 26) run<I + 1>(f);
+  }
+};
+
+// Ghost copies of a face cell: a consumer is clamped onto this cell on axis a
+// when x_a - c_a leaves the domain (c_a = +1 on the low face, -1 on the high
+// face); its load slot on that axis is at p(-c_a) instead of p(c_a).
+template <class S, int P>
+struct TwistGhost {
+  char* base;
+  int64_t dsb;
+  TwistRows r;
+  float s;
+  uint32_t fx, fy, fz;  // bit 0: coordinate 0 on the axis; bit 1: coordinate n-1
+  template <int I>
+  __device__ __forceinline__ void run(const float* f) const {
+    constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
+    constexpr int slot = P == 1 ? 26 - I : I;
+    constexpr int32_t B = sizeof(typename S::T);
+    const uint32_t gx = cx == 1 ? (fx & 1u) : (cx == -1 ? (fx >> 1) : 0u);
+    const uint32_t gy = cy == 1 ? (fy & 1u) : (cy == -1 ? (fy >> 1) : 0u);
+    const uint32_t gz = cz == 1 ? (fz & 1u) : (cz == -1 ? (fz >> 1) : 0u);
+    const uint32_t gm = gx | (gy << 1) | (gz << 2);
+    if (gm) {
+      const typename S::T v = S::enc(f[I], s);
+      char* p = base + slot * dsb;
+      const int32_t sx = cx == 1 ? B : 0, sy = cy == 1 ? r.row : 0, sz = cz == 1 ? r.plane : 0;
+      const int32_t qx = cx == -1 ? B : 0, qy = cy == -1 ? r.row : 0, qz = cz == -1 ? r.plane : 0;
+      for (uint32_t m = 1; m < 8; ++m)
+        if ((m & ~gm) == 0)
+          st_at<typename S::T>(p, r.lb + uint32_t(((m & 1u) ? qx : sx) + ((m & 2u) ? qy : sy) +
+                                                  ((m & 4u) ? qz : sz)), v);
     }
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
 
 template <class S, int P>
-__device__ __forceinline__ TwistStore<S, P> make_twist_store(const StepArgs& a, const Cell& k,
-                                                             void* dst) {
-  TwistStore<S, P> w;
-  w.base = static_cast<typename S::T*>(dst);
-  w.ds = a.dstride;
-  w.sx = 1;
-  w.sy = a.px;
-  w.sz = a.px * a.py;
-  w.l = k.l;
-  w.s = a.scale;
-  w.fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
-  w.fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
-  w.fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
-  w.face = (w.fx | w.fy | w.fz) != 0;
-  return w;
+__device__ __forceinline__ void twist_load_all(const StepArgs& a, const Cell& k, const void* buf,
+                                               float* f) {
+  TwistLoad<S, P> t;
+  t.base = static_cast<const char*>(buf);
+  t.dsb = a.dstride * int64_t(sizeof(typename S::T));
+  t.r = twist_rows<sizeof(typename S::T)>(a, k);
+  t.inv_s = a.inv_scale;
+  t.template run<0>(f);
 }
 
 template <class S, int P>
-__device__ __forceinline__ TwistLoad<S, P> make_twist_load(const StepArgs& a, const Cell& k,
-                                                           const void* src) {
-  TwistLoad<S, P> r;
-  r.base = static_cast<const typename S::T*>(src);
-  r.ds = a.dstride;
-  r.sx = 1;
-  r.sy = a.px;
-  r.sz = a.px * a.py;
-  r.l = k.l;
-  r.inv_s = a.inv_scale;
-  return r;
+__device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k, void* buf,
+                                                const float* f) {
+  TwistStore<S, P> t;
+  t.base = static_cast<char*>(buf);
+  t.dsb = a.dstride * int64_t(sizeof(typename S::T));
+  t.r = twist_rows<sizeof(typename S::T)>(a, k);
+  t.s = a.scale;
+  t.template run<0>(f);
+  const uint32_t fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
+  const uint32_t fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
+  const uint32_t fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
+  if (fx | fy | fz) {  // one branch per wave; interior waves skip it entirely
+    TwistGhost<S, P> g;
+    g.base = t.base;
+    g.dsb = t.dsb;
+    g.r = t.r;
+    g.s = a.scale;
+    g.fx = fx;
+    g.fy = fy;
+    g.fz = fz;
+    g.template run<0>(f);
+  }
 }
 
 template <class S, int P>
 __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
   if (!k.ok) return;
-  const uint32_t fl = a.sflag[k.c];
+  // twisted loads stay inside the padded layout for every cell: issue them
+  // before the flag arrives
   float f[27];
-#pragma unroll
-  for (int i = 0; i < 27; ++i) f[i] = 0.0f;
-  if (fl & kLoad) make_twist_load<S, P>(a, k, a.dst).template run<0>(f);
+  twist_load_all<S, P>(a, k, a.dst, f);
+  const uint32_t fl = a.sflag[k.c];
+  if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, f);
-  if (fl & kStore) make_twist_store<S, P>(a, k, a.dst).template run<0>(f);
-}
-
-// ---------------------------------------------------------------------------
-// cooperative wavefront: all chunk steps of one temporal-blocking pass in a
-// single launch, grid-wide barrier between dependent steps.
-
-struct WaveArgs {
-  StepArgs a;        // z0/z1/ncells are set per chunk step inside the kernel
-  int32_t depth;     // planes per chunk
-  int32_t nsteps;    // steps of this pass
-  int32_t parity0;   // in-place: parity of the first step; pull: unused
-  const void* buf0;  // pull: buffer holding the pass input
-  void* buf1;        // pull: the other buffer
-};
-
-// One cell of a step: load by layout, update, store by layout.
-template <class S, bool INPLACE, int P>
-__device__ __forceinline__ void step_cell(const StepArgs& a, const Cell& k) {
-  const uint32_t fl = a.sflag[k.c];
-  float f[27];
-#pragma unroll
-  for (int i = 0; i < 27; ++i) f[i] = 0.0f;
-  if constexpr (INPLACE) {
-    if (fl & kLoad) make_twist_load<S, P>(a, k, a.dst).template run<0>(f);
-    update_cell(fl, a, k.c, f);
-    if (fl & kStore) make_twist_store<S, P>(a, k, a.dst).template run<0>(f);
-  } else {
-    if (fl & kLoad) make_gather<S>(a, k, a.src).template run<0>(f);
-    update_cell(fl, a, k.c, f);
-    if (fl & kStore)
-      store_plain<S, 0>(static_cast<typename S::T*>(a.dst), a.dstride, k.l, f, a.scale);
-  }
-}
-
-template <class S, bool INPLACE>
-__global__ __launch_bounds__(kBlock) void k_wave_coop(WaveArgs w) {
-  namespace cg = cooperative_groups;
-  cg::grid_group grid = cg::this_grid();
-  const int32_t nz = w.a.nz;
-  const uint32_t plane_cells = uint32_t(w.a.nx) * uint32_t(w.a.ny);
-  const int32_t nchunk = (nz + w.nsteps - 1 + w.depth - 1) / w.depth;
-  const uint32_t nblk = gridDim.x;
-  const uint32_t vblk = xcd_block(blockIdx.x, nblk);
-  for (int32_t k = 0; k < nchunk; ++k) {
-    for (int32_t t = 1; t <= w.nsteps; ++t) {
-      const int32_t lo = max(0, k * w.depth - (t - 1));
-      const int32_t hi = min(nz, (k + 1) * w.depth - (t - 1));
-      if (lo < hi) {
-        StepArgs a = w.a;
-        a.z0 = lo;
-        a.z1 = hi;
-        a.ncells = uint32_t(hi - lo) * plane_cells;
-        const int par = (w.parity0 + t - 1) & 1;
-        if (!INPLACE) {
-          const bool odd = ((t - 1) & 1) != 0;
-          a.src = odd ? w.buf1 : w.buf0;
-          a.dst = odd ? const_cast<void*>(w.buf0) : w.buf1;
-        }
-        const uint32_t nround = (a.ncells + nblk * kBlock - 1) / (nblk * kBlock);
-        for (uint32_t r = 0; r < nround; ++r) {
-          const Cell kc = locate(a, r * nblk + vblk);
-          if (kc.ok) {
-            if (!INPLACE || par == 0)
-              step_cell<S, INPLACE, 0>(a, kc);
-            else
-              step_cell<S, INPLACE, 1>(a, kc);
-          }
-        }
-      }
-      grid.sync();
-    }
-  }
+  if (fl & kStore) twist_store_all<S, P>(a, k, a.dst, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,8 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_dump(const DumpArgs d) {
   if (!k.ok) return;
   float f[27];
   if constexpr (MODE == 0) make_gather<S>(d.a, k, d.a.src).template run<0>(f);
-  if constexpr (MODE == 1) make_twist_load<S, 1>(d.a, k, d.a.src).template run<0>(f);
-  if constexpr (MODE == 2) make_twist_load<S, 0>(d.a, k, d.a.src).template run<0>(f);
+  if constexpr (MODE == 1) twist_load_all<S, 1>(d.a, k, d.a.src, f);
+  if constexpr (MODE == 2) twist_load_all<S, 0>(d.a, k, d.a.src, f);
   const Moments m = moments(f);
   const uint32_t fl = d.flag[k.c];
   if (fl & kDumpRho) d.rho[k.c] = m.rho;
@@ -356,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_to_pull(const StepArgs a, cons
   float f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) f[i] = plain[i * n + k.c];
-  store_plain<S, 0>(static_cast<typename S::T*>(out), a.dstride, k.l, f, a.scale);
+  store_plain_all<S>(a, k, out, f);
 }
 
 template <class S>
@@ -380,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_to_twist(const StepArgs a, con
   float f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) f[i] = plain[i * n + k.c];
-  make_twist_store<S, P>(a, k, out).template run<0>(f);
+  twist_store_all<S, P>(a, k, out, f);
 }
 
 // in-place layout left by a step of parity P -> plain FP32 (regular slots)
@@ -504,36 +489,5 @@ __global__ __launch_bounds__(kBlock) void k_bw_copy_shift(const float* __restric
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
     out[i + shift] = in[i];
 }
-
-// ---------------------------------------------------------------------------
-// explicit instantiation table used by the host dispatcher
-
-template <class S>
-struct KernelSet {
-  static void* pull() { return reinterpret_cast<void*>(&k_pull<S>); }
-  static void* twist(int p) {
-    return p ? reinterpret_cast<void*>(&k_twist<S, 1>) : reinterpret_cast<void*>(&k_twist<S, 0>);
-  }
-  static void* wave(bool inplace) {
-    return inplace ? reinterpret_cast<void*>(&k_wave_coop<S, true>)
-                   : reinterpret_cast<void*>(&k_wave_coop<S, false>);
-  }
-  static void* dump(int mode) {
-    return mode == 0 ? reinterpret_cast<void*>(&k_dump<S, 0>)
-                     : (mode == 1 ? reinterpret_cast<void*>(&k_dump<S, 1>)
-                                  : reinterpret_cast<void*>(&k_dump<S, 2>));
-  }
-  static void* plain_to_pull() { return reinterpret_cast<void*>(&k_plain_to_pull<S>); }
-  static void* pull_to_plain() { return reinterpret_cast<void*>(&k_pull_to_plain<S>); }
-  static void* plain_to_twist(int p) {
-    return p ? reinterpret_cast<void*>(&k_plain_to_twist<S, 1>)
-             : reinterpret_cast<void*>(&k_plain_to_twist<S, 0>);
-  }
-  static void* twist_to_plain(int p) {
-    return p ? reinterpret_cast<void*>(&k_twist_to_plain<S, 1>)
-             : reinterpret_cast<void*>(&k_twist_to_plain<S, 0>);
-  }
-  static void* codec() { return reinterpret_cast<void*>(&k_codec<S>); }
-};
 
 }  // namespace culbm
