@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/culbm.h"
+#include "blocking.hpp"
 #include "lbm_kernels.hip"
 
 using namespace culbm;
@@ -38,6 +39,7 @@ struct culbm_ctx {
   int64_t step = 0;
   float scale = 1.0f, inv_scale = 1.0f;
   int32_t wave_depth = 0;
+  int32_t tiles_x = 0, tiles_y = 0, nchunk = 0;  // mixed policy launch shape
   // static policy: one instantiated graph per starting state of a full pass
   // (pull: index of the current buffer; in-place: parity of the next step)
   hipGraphExec_t pass_graph[2] = {nullptr, nullptr};
@@ -137,9 +139,9 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
                c->stream_policy);
   if (c->opt_policy < CULBM_OPT_NONE || c->opt_policy > CULBM_OPT_DYNAMIC)
     return bad(CULBM_ERR_INVALID, "optPolicy must be in 0..3, got %d", c->opt_policy);
-  if (c->opt_policy == CULBM_OPT_MIXED)
+  if (c->opt_policy == CULBM_OPT_MIXED && c->stream_policy != CULBM_STREAM_PULL)
     return bad(CULBM_ERR_UNSUPPORTED,
-               "optPolicy 2 (HALO_BLOCKING_L1L2) is not provided by this engine");
+               "optPolicy 2 (HALO_BLOCKING_L1L2) runs with --streamPolicy 0 only");
   // same rule as the reference (simulator_expt_platform.cu:365-368)
   if (c->use_ring &&
       (c->stream_policy != CULBM_STREAM_INPLACE || c->opt_policy != CULBM_OPT_DYNAMIC))
@@ -147,6 +149,19 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
                "--useRingDDFBuf is unimplemented unless --streamPolicy 1 --optPolicy 3.");
   if (blocked(*c) && c->inner_loop < 1)
     return bad(CULBM_ERR_INVALID, "innerLoop must be >= 1, got %d", c->inner_loop);
+  if (blocked(*c)) {
+    // the reference's tile check on blockDim*gridDim (simulator_expt_platform.cu:523-534),
+    // kept so that a command line it rejects is rejected here too
+    const int32_t dom[3] = {c->nx, c->ny, c->nz};
+    for (int k = 0; k < 3; ++k) {
+      if (c->block_dim[k] < 1 || c->grid_dim[k] < 1)
+        return bad(CULBM_ERR_INVALID, "blockDim and gridDim entries must be >= 1");
+      const uint64_t tile = uint64_t(c->block_dim[k]) * uint64_t(c->grid_dim[k]);
+      if (tile > 0xffffffffull ||
+          !blk_axis_valid_ref(uint32_t(dom[k]), uint32_t(tile), uint32_t(c->inner_loop)))
+        return bad(CULBM_ERR_INVALID, "<Domain Dimension, Blocking Dimension and InnerLoop> don't match!");
+    }
+  }
   if (c->storage < CULBM_STORAGE_FP32 || c->storage > CULBM_STORAGE_BF16)
     return bad(CULBM_ERR_INVALID, "storage must be 0 (fp32), 1 (fp16) or 2 (bf16), got %d", c->storage);
   if (c->storage == CULBM_STORAGE_FP32 && c->storage_scale != 0.0f && c->storage_scale != 1.0f)
@@ -316,6 +331,36 @@ int static_pass(culbm_ctx* ctx, int32_t T) {
   return CULBM_OK;
 }
 
+// Mixed blocking: two steps per launch with the intermediate in LDS, and one
+// plain step for an odd remainder.
+template <class S>
+int fused_pass(culbm_ctx* ctx, int32_t T) {
+  FusedArgs fa{};
+  fa.a = base_args(ctx);
+  fa.tiles_x = ctx->tiles_x;
+  fa.tiles_y = ctx->tiles_y;
+  fa.zchunk = ctx->wave_depth;
+  fa.nchunk = ctx->nchunk;
+  const uint32_t nblk = uint32_t(ctx->tiles_x) * ctx->tiles_y * ctx->nchunk;
+  for (int32_t s = 0; s + 1 < T; s += 2) {
+    fa.a.src = ctx->buf[ctx->cur];
+    fa.a.dst = ctx->buf[ctx->cur ^ 1];
+    if (int rc = prof_begin(ctx)) return rc;
+    hipLaunchKernelGGL((k_fused2<S>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
+    HIPCHK(ctx, hipGetLastError());
+    if (int rc = prof_end(ctx)) return rc;
+    ctx->cur ^= 1;
+  }
+  if (T & 1) {
+    StepArgs a = base_args(ctx);
+    a.src = ctx->buf[ctx->cur];
+    a.dst = ctx->buf[ctx->cur ^ 1];
+    if (int rc = launch_step<S>(ctx, a, 0, ctx->cfg.nz, 0)) return rc;
+    ctx->cur ^= 1;
+  }
+  return CULBM_OK;
+}
+
 template <class S>
 int do_steps(culbm_ctx* ctx, int32_t nsteps) {
   const culbm_config& c = ctx->cfg;
@@ -335,7 +380,9 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
     int32_t left = nsteps;
     while (left > 0) {
       const int32_t T = std::min(left, c.inner_loop);
-      const int rc = c.opt_policy == CULBM_OPT_STATIC ? static_pass<S>(ctx, T) : wave_pass<S>(ctx, T);
+      const int rc = c.opt_policy == CULBM_OPT_STATIC  ? static_pass<S>(ctx, T)
+                     : c.opt_policy == CULBM_OPT_MIXED ? fused_pass<S>(ctx, T)
+                                                       : wave_pass<S>(ctx, T);
       if (rc) return rc;
       left -= T;
     }
@@ -532,6 +579,42 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
   if (blocked(c)) {
     ctx->wave_depth = c.wave_depth > 0 ? std::min(c.wave_depth, c.nz) : auto_depth(ctx);
   }
+  if (c.opt_policy == CULBM_OPT_MIXED) {
+    // z-chunks per tile column: balance the workgroup count over the resident
+    // workgroup slots (LDS-limited) against the two extra A planes a chunk
+    // recomputes
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c.device) == hipSuccess) ncu = prop.multiProcessorCount;
+    int per_cu = 1;
+    by_storage(c.storage, [&](auto s) {
+      using S = decltype(s);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused2<S>),
+                                                       kFBlock, 0) != hipSuccess)
+        per_cu = 1;
+      return 0;
+    });
+    const int slots = ncu * std::max(per_cu, 1);
+    ctx->tiles_x = (c.nx + kFTX - 1) / kFTX;
+    ctx->tiles_y = (c.ny + kFTY - 1) / kFTY;
+    const double tiles = double(ctx->tiles_x) * ctx->tiles_y;
+    if (c.wave_depth > 0) {
+      ctx->wave_depth = std::min(c.wave_depth, c.nz);
+    } else {
+      double best = 1e30;
+      for (int n = 1; n <= std::min(64, c.nz); ++n) {
+        const int len = (c.nz + n - 1) / n;
+        const double wgs = tiles * n;
+        const double rounds = std::ceil(wgs / slots);
+        const double cost = rounds / (wgs / slots) * (1.0 + 2.0 / len);
+        if (cost < best - 1e-9) {
+          best = cost;
+          ctx->wave_depth = len;
+        }
+      }
+    }
+    ctx->nchunk = (c.nz + ctx->wave_depth - 1) / ctx->wave_depth;
+  }
 
   // reference defaults: all FLUID + DUMP, rho 1, u 0, rest populations
   {
@@ -574,6 +657,16 @@ int culbm_set_state(culbm_ctx* ctx, const uint32_t* flag, const float* rho, cons
     hipLaunchKernelGGL(k_compact_flags, dim3(nblk), dim3(kBlock), 0, ctx->stream, ctx->flag,
                        ctx->sflag, uint64_t(nb));
     HIPCHK(ctx, hipGetLastError());
+    if (!ctx->inplace && ctx->buf[1]) {
+      StepArgs a = full_args(ctx);
+      const int rc = by_storage(ctx->cfg.storage, [&](auto s) {
+        using S = decltype(s);
+        hipLaunchKernelGGL((k_sync_nostore<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                           ctx->buf[ctx->cur], ctx->buf[ctx->cur ^ 1]);
+        return hipGetLastError() == hipSuccess ? CULBM_OK : CULBM_ERR_HIP;
+      });
+      if (rc) return fail(ctx, rc, "buffer sync kernel launch failed");
+    }
   }
   const float* src[4] = {rho, vx, vy, vz};
   float* dst[4] = {ctx->rho, ctx->vx, ctx->vy, ctx->vz};
@@ -827,6 +920,21 @@ int culbm_probe_codec(int32_t device, int32_t storage, float scale, const float*
   if (ddec) (void)hipFree(ddec);
   if (denc) (void)hipFree(denc);
   return rc;
+}
+
+int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, int32_t* starts,
+                        int32_t max_starts) {
+  if (!num) return fail(nullptr, CULBM_ERR_INVALID, "num is NULL");
+  if (!blk_axis_valid(dom, blk, iter))
+    return fail(nullptr, CULBM_ERR_INVALID,
+                "axis of %d cells cannot be cut into tiles of %d advancing %d steps", dom, blk, iter);
+  const int64_t n = blk_num(dom, blk, iter);
+  *num = int32_t(n);
+  if (starts) {
+    if (max_starts < n + 1) return fail(nullptr, CULBM_ERR_INVALID, "starts needs %lld entries", (long long)(n + 1));
+    for (int64_t k = 0; k <= n; ++k) starts[k] = int32_t(blk_valid_start(k, blk, n, iter, dom));
+  }
+  return CULBM_OK;
 }
 
 int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs) {

@@ -298,6 +298,165 @@ __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Two steps per launch with the intermediate populations kept in LDS (the
+// mixed L1/L2 blocking of the reference, HALO_BLOCKING_L1L2,
+// kernel.cu:138-329, redesigned for the 160 KB LDS of a gfx950 CU).
+//
+// A workgroup owns a TX x TY tile of cells and marches along z over a chunk
+// of planes.  For each plane it computes the first step on the tile plus a
+// one-cell xy halo ("A" cells, pulled from src in HBM) into a 3-plane LDS ring,
+// then the second step of the tile ("B" cells, pulled from the ring) into dst.
+// HBM sees the first step's reads (tile + halo) and the second step's writes
+// only.  The intermediate is kept in the storage format, exactly as a plain
+// step would have stored it, so two fused steps equal two plain steps bit for
+// bit.  A cell without STORE keeps its populations in both buffers (see
+// culbm_set_state), so its intermediate value is its src value.
+
+constexpr int kFTX = 32, kFTY = 8;
+constexpr int kFAX = kFTX + 2, kFAY = kFTY + 2, kFA = kFAX * kFAY;  // 34 x 10 = 340 A cells
+constexpr int kFBlock = 384;  // six waves: the 340 A cells of a plane in one round
+
+struct FusedArgs {
+  StepArgs a;
+  int32_t tiles_x, tiles_y;
+  int32_t zchunk;  // B planes per workgroup
+  int32_t nchunk;
+};
+
+// LDS of one workgroup.  A plane's first-step populations are grouped by c_z:
+// the c_z = +1 group is read by the B step of the next plane, the c_z = 0 group
+// by the same plane, the c_z = -1 group by the previous plane, so the groups
+// live 3, 2 and 1 plane(s): 54 values per A cell instead of 81.
+template <class T>
+struct FusedRing {
+  T up[3][9][kFA];  // directions 18..26 (c_z = +1), slot = plane % 3
+  T mid[2][9][kFA]; // directions 9..17 (c_z = 0), slot = plane % 2
+  T dn[9][kFA];     // directions 0..8 (c_z = -1), the latest plane
+};
+
+// Pulled (not yet updated) populations of one A cell, gathered a plane ahead
+// so the HBM latency overlaps the previous plane's second step.
+struct APending {
+  float f[27];
+  uint32_t fl;
+  uint32_t c;
+  bool ok;
+};
+
+template <class S>
+__device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, int32_t y0, int ac,
+                                               int32_t p, APending& q) {
+  const int32_t gx = x0 - 1 + ac % kFAX, gy = y0 - 1 + ac / kFAX;
+  q.ok = gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny;
+  if (!q.ok) return;
+  Cell k;
+  k.x = gx;
+  k.y = gy;
+  k.z = p;
+  k.c = (uint32_t(p) * uint32_t(a.ny) + uint32_t(gy)) * uint32_t(a.nx) + uint32_t(gx);
+  k.l = k.c;
+  q.c = k.c;
+  make_gather<S>(a, k, a.src).template run<0>(q.f);
+  q.fl = a.sflag[k.c];
+}
+
+// First step of a gathered A cell into the ring: a plain pull update, or the
+// cell's own populations when it has no STORE.
+template <class S>
+__device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_t p, APending& q,
+                                               FusedRing<typename S::T>& r) {
+  using T = typename S::T;
+  if (!q.ok) return;
+  if (!(q.fl & kLoad)) clear(q.f);
+  update_cell(q.fl, a, q.c, q.f);
+  T v[27];
+  if (q.fl & kStore) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) v[i] = S::enc(q.f[i], a.scale);
+  } else {
+    constexpr uint32_t B = sizeof(T);
+    const char* src = static_cast<const char*>(a.src);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) v[i] = ld_at<T>(src + i * (a.dstride * B), q.c * B);
+  }
+  const int su = p % 3, sm = p % 2;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.dn[j][ac] = v[j];
+    r.mid[sm][j][ac] = v[9 + j];
+    r.up[su][j][ac] = v[18 + j];
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
+  using T = typename S::T;
+  __shared__ FusedRing<T> r;
+  const uint32_t ntile = uint32_t(fa.tiles_x) * fa.tiles_y * fa.nchunk;
+  const uint32_t w = xcd_block(blockIdx.x, ntile);
+  const int32_t tx = int32_t(w % uint32_t(fa.tiles_x));
+  const int32_t ty = int32_t((w / uint32_t(fa.tiles_x)) % uint32_t(fa.tiles_y));
+  const int32_t zc = int32_t(w / (uint32_t(fa.tiles_x) * fa.tiles_y));
+  const int32_t x0 = tx * kFTX, y0 = ty * kFTY;
+  const int32_t nx = fa.a.nx, ny = fa.a.ny, nz = fa.a.nz;
+  const int32_t zs = zc * fa.zchunk, ze = min(nz, zs + fa.zchunk);
+  const int t = threadIdx.x;
+  const bool a_ok = t < kFA;
+
+  APending q;
+  q.ok = false;
+  // prologue: A planes zs-1 (if any) and zs, then the gather of plane zs+1
+  for (int32_t p = max(zs - 1, 0); p <= zs; ++p) {
+    if (a_ok) {
+      fused_a_gather<S>(fa.a, x0, y0, t, p, q);
+      fused_a_finish<S>(fa.a, t, p, q, r);
+    }
+    __syncthreads();  // the dn group of plane zs-1 is overwritten by plane zs
+  }
+  if (a_ok && zs + 1 < nz) fused_a_gather<S>(fa.a, x0, y0, t, zs + 1, q);
+
+  const int bx = t % kFTX, by = t / kFTX;
+  const int32_t gx = x0 + bx, gy = y0 + by;
+  const bool b_ok = t < kFTX * kFTY && gx < nx && gy < ny;
+  // local A coordinates of the (clamped) source columns
+  const int lxm = gx == 0 ? bx + 1 : bx, lxp = gx == nx - 1 ? bx + 1 : bx + 2;
+  const int lym = gy == 0 ? by + 1 : by, lyp = gy == ny - 1 ? by + 1 : by + 2;
+  for (int32_t z = zs; z < ze; ++z) {
+    if (a_ok && z + 1 < nz) {
+      fused_a_finish<S>(fa.a, t, z + 1, q, r);
+      if (z + 1 < ze && z + 2 < nz) fused_a_gather<S>(fa.a, x0, y0, t, z + 2, q);
+    }
+    __syncthreads();
+    if (b_ok) {
+      // c_z = +1 pulls from plane z-1 (clamped to z), c_z = -1 from z+1 (the
+      // dn group still holds plane z when z is the last plane)
+      const int su = (z == 0 ? z : z - 1) % 3, sm = z % 2;
+      float f[27];
+#pragma unroll
+      for (int i = 0; i < 27; ++i) {
+        const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
+        const int lx = cx == 1 ? lxm : (cx == -1 ? lxp : bx + 1);
+        const int ly = cy == 1 ? lym : (cy == -1 ? lyp : by + 1);
+        const int a = ly * kFAX + lx;
+        const T v = cz == 1 ? r.up[su][i - 18][a] : (cz == 0 ? r.mid[sm][i - 9][a] : r.dn[i][a]);
+        f[i] = S::dec(v, fa.a.inv_scale);
+      }
+      Cell k;
+      k.x = gx;
+      k.y = gy;
+      k.z = z;
+      k.c = (uint32_t(z) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx);
+      k.l = k.c;
+      const uint32_t fl = fa.a.sflag[k.c];
+      if (!(fl & kLoad)) clear(f);
+      update_cell(fl, fa.a, k.c, f);
+      if (fl & kStore) store_plain_all<S>(fa.a, k, fa.a.dst, f);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // dump: rho/u of the populations the next step would load, written only
 // where the DUMP bits are set (kernel.cu:2634-2735).  mode 0: pull layout;
 // mode 1: in-place layout awaiting an O step; mode 2: awaiting an E step.
@@ -401,6 +560,23 @@ __global__ __launch_bounds__(kBlock) void k_feq_plain(const StepArgs a, float* p
 __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, uint64_t nwords) {
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nwords; i += stride) p[i] = 0u;
+}
+
+// Cells without STORE never change; keep their two pull buffers identical
+// (copy the current buffer into the other) so that any order of buffer reads
+// sees the same value.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sync_nostore(const StepArgs a, const void* cur,
+                                                         void* other) {
+  const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
+  if (!k.ok || (a.sflag[k.c] & kStore)) return;
+  constexpr uint32_t B = sizeof(typename S::T);
+  const char* s = static_cast<const char*>(cur);
+  char* d = static_cast<char*>(other);
+  const int64_t dsb = a.dstride * B;
+#pragma unroll
+  for (int i = 0; i < 27; ++i)
+    st_at<typename S::T>(d + i * dsb, k.l * B, ld_at<typename S::T>(s + i * dsb, k.l * B));
 }
 
 // Compact per-cell step flags.

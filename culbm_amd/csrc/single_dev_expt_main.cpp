@@ -33,6 +33,7 @@ struct Options {
   bool dump_rho = false, dump_vx = false, dump_vy = false, dump_vz = false;
   int32_t init_ddf = CULBM_INIT_REST;
   bool json = false;
+  bool dry_run = false;
 };
 
 struct ParseError : std::runtime_error {
@@ -64,7 +65,8 @@ const char* kUsage =
     "  --storageScale FLOAT      16-bit storage scale (power of two, 0 = default) [0]\n"
     "  --waveDepth INT           Planes per temporal-blocking chunk (0 = automatic) [0]\n"
     "  --initDDF TEXT            Populations at start {rest,feq} [rest]\n"
-    "  --json                    Also print one JSON line per window\n";
+    "  --json                    Also print one JSON line per window\n"
+    "  --dryRun                  Validate, print the parsed configuration as JSON and exit\n";
 
 long parse_int(const std::string& opt, const std::string& v) {
   char* end = nullptr;
@@ -209,6 +211,8 @@ Options parse(int argc, char** argv) {
       else throw ParseError("--initDDF: expected rest or feq, got " + v);
     } else if (opt == "--json") {
       o.json = flag();
+    } else if (opt == "--dryRun") {
+      o.dry_run = flag();
     } else {
       throw ParseError("The following argument was not expected: " + args[k]);
     }
@@ -234,37 +238,58 @@ class Simulator {
     char msg[512];
     const int v = culbm_validate(&o_.cfg, msg, sizeof(msg));
     if (v != CULBM_OK) throw std::runtime_error(msg);
-    check(culbm_create(&o_.cfg, &ctx_), nullptr);
+    // host inputs first (initFlag / initRhoU, simulator_expt_platform.cu:95-118, 200-228),
+    // so a missing flag.dat is reported before any device work
     const size_t n = size_t(o_.cfg.nx) * o_.cfg.ny * o_.cfg.nz;
-    std::vector<uint32_t> flag;
     if (!o_.init_folder.empty()) {
-      flag.resize(n);
-      if (!read_file(o_.init_folder + "/flag.dat", flag))
+      flag_.resize(n);
+      if (!read_file(o_.init_folder + "/flag.dat", flag_))
         throw std::runtime_error("File: " + o_.init_folder + "/flag.dat doesn't exist!");
-      std::cout << "Init flag successfully, from file: " << o_.init_folder << "/flag.dat." << std::endl;
     }
     const char* names[4] = {"rho", "vx", "vy", "vz"};
-    const char* dflt[4] = {"1", "0", "0", "0"};
-    std::vector<float> fields[4];
     for (int k = 0; k < 4; ++k) {
-      const std::string path = o_.init_folder + "/" + names[k] + ".dat";
-      fields[k].resize(n);
-      if (!o_.init_folder.empty() && read_file(path, fields[k])) {
-        std::cout << "Init " << names[k] << " successfully, from file: " << path << "." << std::endl;
-      } else {
-        fields[k].clear();
+      fields_[k].resize(n);
+      from_file_[k] = !o_.init_folder.empty() &&
+                      read_file(o_.init_folder + "/" + names[k] + ".dat", fields_[k]);
+      if (!from_file_[k]) fields_[k].clear();
+    }
+  }
+
+  void init() {
+    check(culbm_create(&o_.cfg, &ctx_), nullptr);
+    if (!flag_.empty())
+      std::cout << "Init flag successfully, from file: " << o_.init_folder << "/flag.dat." << std::endl;
+    const char* names[4] = {"rho", "vx", "vy", "vz"};
+    const char* dflt[4] = {"1", "0", "0", "0"};
+    for (int k = 0; k < 4; ++k) {
+      if (from_file_[k])
+        std::cout << "Init " << names[k] << " successfully, from file: " << o_.init_folder << "/"
+                  << names[k] << ".dat." << std::endl;
+      else
         std::cout << "Init " << names[k] << " successfully, default init with value " << dflt[k]
                   << "." << std::endl;
-      }
     }
     auto p = [](std::vector<float>& v) { return v.empty() ? nullptr : v.data(); };
-    check(culbm_set_state(ctx_, flag.empty() ? nullptr : flag.data(), p(fields[0]), p(fields[1]),
-                          p(fields[2]), p(fields[3])),
+    check(culbm_set_state(ctx_, flag_.empty() ? nullptr : flag_.data(), p(fields_[0]), p(fields_[1]),
+                          p(fields_[2]), p(fields_[3])),
           ctx_);
     check(culbm_init_ddf(ctx_, o_.init_ddf), ctx_);
     std::cout << (o_.cfg.use_ring ? "Init ring ddf buffer successfully."
                                   : "Init double ddf buffer successfully.")
               << std::endl;
+  }
+
+  void print_config() const {
+    const culbm_config& c = o_.cfg;
+    std::printf(
+        "{\"domainDim\": [%d, %d, %d], \"blockDim\": [%d, %d, %d], \"gridDim\": [%d, %d, %d], "
+        "\"invTau\": %.9g, \"streamPolicy\": %d, \"optPolicy\": %d, \"innerLoop\": %d, "
+        "\"nstep\": %d, \"dstep\": %d, \"devId\": %d, \"useRingDDFBuf\": %s, \"storage\": %d, "
+        "\"initStateFolder\": \"%s\", \"dumpFolder\": \"%s\", \"dump\": [%d, %d, %d, %d]}\n",
+        c.nx, c.ny, c.nz, c.block_dim[0], c.block_dim[1], c.block_dim[2], c.grid_dim[0], c.grid_dim[1],
+        c.grid_dim[2], double(c.inv_tau), c.stream_policy, c.opt_policy, c.inner_loop, o_.nstep,
+        o_.dstep, c.device, c.use_ring ? "true" : "false", c.storage, o_.init_folder.c_str(),
+        o_.dump_folder.c_str(), int(o_.dump_rho), int(o_.dump_vx), int(o_.dump_vy), int(o_.dump_vz));
   }
   ~Simulator() { culbm_destroy(ctx_); }
 
@@ -315,6 +340,9 @@ class Simulator {
 
   Options o_;
   culbm_ctx* ctx_ = nullptr;
+  std::vector<uint32_t> flag_;
+  std::vector<float> fields_[4];
+  bool from_file_[4] = {false, false, false, false};
 };
 
 }  // namespace
@@ -329,6 +357,11 @@ int main(int argc, char** argv) {
   }
   try {
     Simulator sim(o);
+    if (o.dry_run) {
+      sim.print_config();
+      return 0;
+    }
+    sim.init();
     sim.run();
   } catch (const std::exception& e) {
     std::cerr << e.what() << std::endl;

@@ -182,6 +182,13 @@ int culbm_probe_cells(int32_t device, int32_t op, float inv_tau, float* f, const
 int culbm_probe_codec(int32_t device, int32_t storage, float scale, const float* in,
                       uint16_t* enc, float* dec, int64_t n);
 
+/* Halo-blocking decomposition of one axis (src/blocking_core/blocking_alogrithm.hpp:8-41):
+ * *num tiles of length blk advancing iter steps cover [0, dom); starts[k], k = 0..num,
+ * are the valid-range boundaries (starts[0] = 0, starts[num] = dom); tile k loads
+ * [max(starts[k]-(iter-1), 0), min(starts[k+1]+(iter-1), dom)).  starts may be NULL. */
+int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, int32_t* starts,
+                        int32_t max_starts);
+
 /* Bandwidth calibration (counterpart of the reference's analysis/lambda and
  * analysis/gamma probes): mode 0 copy, 1 read, 2 write (16-byte accesses),
  * 3 scalar copy, 4 scalar copy with every store shifted by one element.

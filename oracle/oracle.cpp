@@ -430,19 +430,22 @@ void orc_decode16(int fmt, const uint16_t* in, float* out, int64_t n) {
 }
 
 // ---- halo-blocking axis math (blocking_alogrithm.hpp:8-41) -----------------
-// Restated exactly, including the reference's validity test.
-int orc_blk_valid(int dom, int blk, int iter) {
-  if (blk < dom && blk + 1 <= iter) return 0;
-  if (2 * (blk + 1 - iter) < dom && blk + 2 <= iter) return 0;
+// Restated exactly, including the reference's validity test and the unsigned
+// 32-bit arithmetic of its call site (getBlockingNumDim, simulator_expt_platform.cu:523-540).
+int orc_blk_valid(int dom_, int blk_, int iter_) {
+  const uint32_t dom = uint32_t(dom_), blk = uint32_t(blk_), iter = uint32_t(iter_);
+  if (blk < dom && blk + 1u <= iter) return 0;
+  if (2u * (blk + 1u - iter) < dom && blk + 2u <= iter) return 0;
   return 1;
 }
 
-int orc_blk_num(int dom, int blk, int iter) {
+int orc_blk_num(int dom_, int blk_, int iter_) {
+  const uint32_t dom = uint32_t(dom_), blk = uint32_t(blk_), iter = uint32_t(iter_);
   if (dom <= blk) return 1;
-  if (dom <= 2 * (blk + 1 - iter)) return 2;
-  const unsigned num = unsigned(dom - 2 * (blk + 1 - iter));
-  const unsigned den = unsigned(blk + 2 - 2 * iter);
-  return 2 + int((num + den - 1) / den);
+  if (dom <= 2u * (blk + 1u - iter)) return 2;
+  const uint32_t num = dom - 2u * (blk + 1u - iter);
+  const uint32_t den = blk + 2u - 2u * iter;
+  return int(2u + (num + den - 1u) / den);
 }
 
 int orc_valid_prev(int idx, int blk, int num, int iter, int dom) {
