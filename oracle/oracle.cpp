@@ -28,6 +28,9 @@
 #include <cstring>
 #include <omp.h>
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
 constexpr uint32_t F_LOAD = 1u << 0;
@@ -455,5 +458,187 @@ int orc_valid_prev(int idx, int blk, int num, int iter, int dom) {
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+// ---- discrete adjoint (FP64; no reference counterpart) ---------------------
+//
+// Forward: f^{t+1}(x) = Phi_x(g^t(x)), g^t = P f^t (the clamped pull), Phi_x the
+// per-cell LOAD/EQU/BB/COLLIDE/STORE update.  Objective
+//   J = sum_{t in [t0, t1]} sum_x w(x) u_x(g^t(x)),
+// the weighted x-velocity of the populations each cell pulls (what the dump
+// kernel reports for the state after t steps).  Adjoint: mu^t(x) = dJ/dg^t(x),
+//   mu^t(x) = (dPhi_x/dg)^T lambda^{t+1}(x) + [t in window] w(x) du_x/dg,
+//   lambda^t = P^T mu^t  (each population summed over the cells that pull it).
+// Parameters: the EQU inputs (rho, u) of every cell and invTau.
+
+// lam_h = (dC/dh)^T lam_o for the BGK collision at pre-collision h;
+// returns sum_i lam_o_i * dC_i/dinvTau = sum_i lam_o_i (feq_i - h_i).
+double adj_collide_f64(double inv_tau, const double* h, const double* lam_o, double* lam_h) {
+  double rho, ux, uy, uz;
+  moments(h, rho, ux, uy, uz);
+  double feq[27];
+  equ<double>(rho, ux, uy, uz, feq);
+  const double u2 = ux * ux + uy * uy + uz * uz;
+  double a = 0.0, bx = 0.0, by = 0.0, bz = 0.0, s = 0.0, dtau = 0.0;
+  for (int i = 0; i < 27; ++i) {
+    const int k = wclass(i);
+    const double w = k == 0 ? 8.0 / 27 : (k == 1 ? 2.0 / 27 : (k == 2 ? 1.0 / 54 : 1.0 / 216));
+    const double cu = cx_of(i) * ux + cy_of(i) * uy + cz_of(i) * uz;
+    const double lw = lam_o[i] * w;
+    a += lw * (1.0 - 4.5 * cu * cu + 1.5 * u2);  // d feq_i / d rho at fixed momentum
+    const double q = lw * (3.0 + 9.0 * cu);      // d feq_i / d m = w (3c + 9(c.u)c - 3u)
+    bx += q * cx_of(i);
+    by += q * cy_of(i);
+    bz += q * cz_of(i);
+    s += lw;
+    dtau += lam_o[i] * (feq[i] - h[i]);
+  }
+  bx -= 3.0 * ux * s;
+  by -= 3.0 * uy * s;
+  bz -= 3.0 * uz * s;
+  for (int j = 0; j < 27; ++j)
+    lam_h[j] = (1.0 - inv_tau) * lam_o[j] + inv_tau * (a + bx * cx_of(j) + by * cy_of(j) + bz * cz_of(j));
+  return dtau;
+}
+
+// grad4 += (d feq / d(rho, ux, uy, uz))^T lam at fixed u (EQU inputs).
+void adj_feq_inputs_f64(double rho, double ux, double uy, double uz, const double* lam, double* grad4) {
+  const double u2 = ux * ux + uy * uy + uz * uz;
+  for (int i = 0; i < 27; ++i) {
+    const int k = wclass(i);
+    const double w = k == 0 ? 8.0 / 27 : (k == 1 ? 2.0 / 27 : (k == 2 ? 1.0 / 54 : 1.0 / 216));
+    const double cu = cx_of(i) * ux + cy_of(i) * uy + cz_of(i) * uz;
+    grad4[0] += lam[i] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+    const double q = lam[i] * w * rho * (3.0 + 9.0 * cu);
+    grad4[1] += q * cx_of(i) - lam[i] * w * rho * 3.0 * ux;
+    grad4[2] += q * cy_of(i) - lam[i] * w * rho * 3.0 * uy;
+    grad4[3] += q * cz_of(i) - lam[i] * w * rho * 3.0 * uz;
+  }
+}
+
+// Pulled populations of cell (x,y,z) from state f (dense [27][n]).
+inline void pull_cell(const Dom& d, const double* f, int x, int y, int z, double* g) {
+  for (int i = 0; i < 27; ++i) g[i] = f[int64_t(i) * d.n + pull_src(d, x, y, z, i)];
+}
+
+// Forward run with all states kept, objective J, adjoint sweep; returns J.
+// EQU inputs are taken as double (rho/ux/uy/uz) so finite differences can
+// perturb them.  grad_* have n entries (accumulated over every step);
+// grad_f0 (27n, may be NULL) receives dJ/df^0.
+double orc_adjoint_f64(int nx, int ny, int nz, const uint32_t* flag, const double* rho_in,
+                       const double* ux_in, const double* uy_in, const double* uz_in,
+                       double inv_tau, int nsteps, int t0, int t1, const double* weight,
+                       const double* f0, double* grad_rho, double* grad_ux, double* grad_uy,
+                       double* grad_uz, double* grad_tau, double* grad_f0) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  const int64_t n = d.n;
+  std::vector<double> states(size_t(27 * n) * size_t(nsteps + 1));
+  std::memcpy(states.data(), f0, sizeof(double) * 27 * n);
+  double J = 0.0;
+  auto cell = [&](int x, int y, int z) { return (int64_t(z) * ny + y) * nx + x; };
+  auto objective = [&](const double* f) {
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t c = cell(x, y, z);
+          if (weight[c] == 0.0) continue;
+          double g[27], r, vx, vy, vz;
+          pull_cell(d, f, x, y, z, g);
+          moments(g, r, vx, vy, vz);
+          s += weight[c] * vx;
+        }
+    return s;
+  };
+  for (int t = 0; t < nsteps; ++t) {
+    const double* src = states.data() + size_t(27 * n) * t;
+    double* dst = states.data() + size_t(27 * n) * (t + 1);
+    if (t >= t0 && t <= t1) J += objective(src);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t c = cell(x, y, z);
+          const uint32_t fl = flag[c];
+          double f[27] = {0};
+          if (fl & F_LOAD) pull_cell(d, src, x, y, z, f);
+          if (fl & F_EQU) equ<double>(rho_in[c], ux_in[c], uy_in[c], uz_in[c], f);
+          if (fl & F_BB) bounce_back(f);
+          if (fl & F_COLLIDE) collide<double>(inv_tau, f);
+          for (int i = 0; i < 27; ++i) dst[int64_t(i) * n + c] = (fl & F_STORE) ? f[i] : src[int64_t(i) * n + c];
+        }
+  }
+  if (nsteps >= t0 && nsteps <= t1) J += objective(states.data() + size_t(27 * n) * nsteps);
+
+  // lam holds lambda^{t+1} = dJ/df^{t+1}; mu holds dJ/dg^t.  A cell without
+  // STORE keeps f^{t+1}(x) = f^t(x) (its populations never change), so its
+  // lambda carries straight over to the previous state.
+  std::vector<double> mu(size_t(27 * n), 0.0), lam(size_t(27 * n), 0.0), nxt(size_t(27 * n), 0.0);
+  double gt = 0.0;
+  for (int64_t c = 0; c < n; ++c) grad_rho[c] = grad_ux[c] = grad_uy[c] = grad_uz[c] = 0.0;
+  auto add_objective = [&](const double* f) {  // mu += w du_x/dg
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t c = cell(x, y, z);
+          if (weight[c] == 0.0) continue;
+          double g[27], r, vx, vy, vz;
+          pull_cell(d, f, x, y, z, g);
+          moments(g, r, vx, vy, vz);
+          for (int i = 0; i < 27; ++i) mu[int64_t(i) * n + c] += weight[c] * (cx_of(i) - vx) / r;
+        }
+  };
+  auto transpose_pull = [&](std::vector<double>& out) {  // out += P^T mu
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+          for (int i = 0; i < 27; ++i)
+            out[int64_t(i) * n + pull_src(d, x, y, z, i)] += mu[int64_t(i) * n + cell(x, y, z)];
+  };
+  // lambda^N
+  if (nsteps >= t0 && nsteps <= t1) add_objective(states.data() + size_t(27 * n) * nsteps);
+  transpose_pull(lam);
+  for (int t = nsteps - 1; t >= 0; --t) {
+    const double* src = states.data() + size_t(27 * n) * t;
+    std::fill(mu.begin(), mu.end(), 0.0);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t c = cell(x, y, z);
+          const uint32_t fl = flag[c];
+          if (!(fl & F_STORE)) continue;
+          double g[27] = {0}, h[27], lo[27], lh[27];
+          if (fl & F_LOAD) pull_cell(d, src, x, y, z, g);
+          for (int i = 0; i < 27; ++i) h[i] = g[i];
+          if (fl & F_EQU) equ<double>(rho_in[c], ux_in[c], uy_in[c], uz_in[c], h);
+          if (fl & F_BB) bounce_back(h);
+          for (int i = 0; i < 27; ++i) lo[i] = lam[int64_t(i) * n + c];
+          if (fl & F_COLLIDE) {
+            gt += adj_collide_f64(inv_tau, h, lo, lh);
+          } else {
+            for (int i = 0; i < 27; ++i) lh[i] = lo[i];
+          }
+          if (fl & F_BB) bounce_back(lh);
+          if (fl & F_EQU) {
+            double g4[4] = {0, 0, 0, 0};
+            adj_feq_inputs_f64(rho_in[c], ux_in[c], uy_in[c], uz_in[c], lh, g4);
+            grad_rho[c] += g4[0];
+            grad_ux[c] += g4[1];
+            grad_uy[c] += g4[2];
+            grad_uz[c] += g4[3];
+          } else if (fl & F_LOAD) {
+            for (int i = 0; i < 27; ++i) mu[int64_t(i) * n + c] = lh[i];
+          }
+        }
+    if (t >= t0 && t <= t1) add_objective(src);
+    std::fill(nxt.begin(), nxt.end(), 0.0);
+    for (int64_t c = 0; c < n; ++c)
+      if (!(flag[c] & F_STORE))
+        for (int i = 0; i < 27; ++i) nxt[int64_t(i) * n + c] = lam[int64_t(i) * n + c];
+    transpose_pull(nxt);
+    std::swap(lam, nxt);  // lambda^t
+  }
+  if (grad_f0) std::memcpy(grad_f0, lam.data(), sizeof(double) * 27 * n);
+  if (grad_tau) *grad_tau = gt;
+  return J;
+}
 
 }  // extern "C"

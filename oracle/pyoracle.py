@@ -230,3 +230,31 @@ class Sim:
                                self.scale, self.src, *out, self.nthreads)
         self.fields = out
         return tuple(out)
+
+
+_F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def adjoint_f64(state, inv_tau, nsteps, t0, t1, weight, f0, rho=None, ux=None, uy=None, uz=None):
+    """FP64 forward + discrete adjoint on the CPU (small grids: every state is
+    kept).  Returns (J, dict of gradients: rho, ux, uy, uz (per cell EQU inputs),
+    inv_tau, f0)."""
+    fn = lib().orc_adjoint_f64
+    fn.restype = ctypes.c_double
+    fn.argtypes = [_i, _i, _i, _U32, _F64P, _F64P, _F64P, _F64P, ctypes.c_double, _i, _i, _i,
+                   _F64P, _F64P, _F64P, _F64P, _F64P, _F64P, ctypes.POINTER(ctypes.c_double), _F64P]
+    nz, ny, nx = state.flag.shape
+    n = nx * ny * nz
+    fields = [np.ascontiguousarray(v if v is not None else getattr(state, k), np.float64).reshape(-1)
+              for v, k in ((rho, "rho"), (ux, "vx"), (uy, "vy"), (uz, "vz"))]
+    g = {k: np.zeros(n) for k in ("rho", "ux", "uy", "uz")}
+    gf0 = np.zeros(27 * n)
+    gt = ctypes.c_double()
+    J = fn(nx, ny, nz, np.ascontiguousarray(state.flag, np.uint32).reshape(-1), *fields, float(inv_tau),
+           int(nsteps), int(t0), int(t1), np.ascontiguousarray(weight, np.float64).reshape(-1),
+           np.ascontiguousarray(f0, np.float64).reshape(-1), g["rho"], g["ux"], g["uy"], g["uz"],
+           ctypes.byref(gt), gf0)
+    g = {k: v.reshape(nz, ny, nx) for k, v in g.items()}
+    g["inv_tau"] = gt.value
+    g["f0"] = gf0.reshape(27, nz, ny, nx)
+    return J, g
