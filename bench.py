@@ -91,6 +91,103 @@ def run_form(state, name, steps, warmup, inner_loop):
     }
 
 
+def outlet_weights(state, t_begin, t_end):
+    """w of J = mean outlet u_x over the states t_begin..t_end: the FLUID cells
+    of the x = nx-1 face, each weighted 1 / (cells * states)."""
+    from culbm_amd import cases
+
+    w = np.zeros(state.shape)
+    fluid = cases.LOAD | cases.COLLIDE | cases.STORE
+    out = (state.flag[:, :, -1] & 0x1f) == fluid
+    w[:, :, -1][out] = 1.0 / (out.sum() * (t_end - t_begin + 1))
+    return w
+
+
+def forward_ms(state, sp, op, storage, steps=100, warmup=10):
+    """Device-timed ms per forward step of one form on `state`."""
+    from culbm_amd import Lattice, LatticeConfig
+
+    lat = Lattice(LatticeConfig(domain=state.domain, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
+                                inner_loop=5, storage=storage))
+    lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
+    lat.init_ddf("rest")
+    lat.step(warmup)
+    ms = lat.time_steps(steps) / steps
+    lat.close()
+    return ms
+
+
+def adjoint_run(state, storage, nsteps, window, **kw):
+    from culbm_amd import Lattice, LatticeConfig
+
+    lat = Lattice(LatticeConfig(domain=state.domain, inv_tau=INV_TAU, storage=storage))
+    lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
+    w = outlet_weights(state, *window)
+    res = None
+    for _ in range(2):  # the second call is measured (the first pays page-in of the workspace)
+        lat.init_ddf("rest")
+        res = lat.adjoint(w, nsteps, window, timing=True, **kw)
+    lat.close()
+    return res
+
+
+def adjoint_ratio(large=False):
+    """Adjoint/forward step ratio on the differentiation workload: 128^3 channel,
+    20 spheres r 4-8, smooth inlet profile, FP32, 300 steps, checkpoints every
+    20 steps, J = mean outlet u_x over steps 251-300.  With `large`, also the
+    256^3 FP16 run (100 spheres r 5-12, 1000 steps, binomial checkpoints under
+    32 GiB) and its FP32-storage counterpart."""
+    from culbm_amd import cases
+
+    st, _ = cases.config_adjoint(128)
+    n = 128 ** 3
+    t_fwd = {"fp32_inplace_blocked": forward_ms(st, 1, 3, "fp32"), "fp32_plain_pull": forward_ms(st, 0, 0, "fp32")}
+    r = adjoint_run(st, "fp32", 300, (251, 300), checkpoint_every=20)
+    s = r.stats
+    rev = s["ms_reverse"] / 300
+    rec = s["ms_recompute"] / 300
+    base = t_fwd["fp32_inplace_blocked"]
+    out = {
+        "workload": "128^3 channel, 20 spheres r 4-8, inlet profile, FP32, 300 steps, "
+                    "checkpoint every 20, J = mean outlet u_x over steps 251-300",
+        "forward_ms_per_step": {k: round(v, 4) for k, v in t_fwd.items()},
+        "reverse_ms_per_step": round(rev, 4),
+        "recompute_ms_per_step": round(rec, 4),
+        "without_recompute": round(rev / base, 3),
+        "with_recompute": round((rev + rec) / base, 3),
+        "whole_call_over_forward": round(s["ms_total"] / (300 * base), 3),
+        "adjoint_kernel_gbs": round(324 * n / (rev * 1e-3) / 1e9, 1),
+        "forward_steps": s["forward_steps"],
+        "device_bytes": s["device_bytes"],
+        "objective": r.objective,
+        "grad_inv_tau": r.grad_inv_tau,
+    }
+    if large:
+        big = cases.config_adjoint(256, seed=5, count=100, r_min=5.0, r_max=12.0)[0]
+        nb = 256 ** 3
+        cap = 32 << 30
+        f16 = adjoint_run(big, "fp16", 1000, (951, 1000), mem_cap=cap)
+        f32 = adjoint_run(big, "fp32", 1000, (951, 1000), mem_cap=cap)
+        t16 = forward_ms(big, 1, 3, "fp16", steps=50, warmup=5)
+        inlet = (big.flag & cases.EQU) != 0
+        g16, g32 = f16.grad_ux[inlet], f32.grad_ux[inlet]
+        out["large"] = {
+            "workload": "256^3 channel, 100 spheres r 5-12, FP16 forward, FP32 adjoint, 1000 steps, "
+                        "binomial checkpoints under 32 GiB, J = mean outlet u_x over steps 951-1000",
+            "forward_fp16_inplace_blocked_ms_per_step": round(t16, 4),
+            "forward_mlups": round(nb / (t16 * 1e-3) / 1e6, 1),
+            "slots": f16.stats["slots"],
+            "forward_steps": f16.stats["forward_steps"],
+            "device_bytes": f16.stats["device_bytes"],
+            "ms": {k: round(f16.stats[k], 1) for k in ("ms_sweep", "ms_recompute", "ms_reverse", "ms_total")},
+            "without_recompute": round(f16.stats["ms_reverse"] / 1000 / t16, 3),
+            "with_recompute": round((f16.stats["ms_reverse"] + f16.stats["ms_recompute"]) / 1000 / t16, 3),
+            "grad_ux_rel_err_vs_fp32_storage": float(np.abs(g16 - g32).max() / np.abs(g32).max()),
+            "objective_rel_err_vs_fp32_storage": abs(f16.objective - f32.objective) / abs(f32.objective),
+        }
+    return out
+
+
 def cpu_baseline(state, budget_s=15.0):
     """FP32 OpenMP restatement of the plain pull step on the same workload."""
     from oracle import pyoracle
@@ -125,6 +222,9 @@ def main(argv=None) -> int:
     ap.add_argument("--inner-loop", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="measure only the headline form")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
+    ap.add_argument("--adjoint-large", action="store_true",
+                    help="also run the 256^3 FP16 adjoint with binomial checkpoints")
     ns = ap.parse_args(argv)
     if ns.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not provided "
@@ -143,6 +243,10 @@ def main(argv=None) -> int:
             r = run_form(state, name, min(ns.steps, 50), min(ns.warmup, 5), ns.inner_loop)
             forms[name] = {k: round(v, 4) if isinstance(v, float) else v for k, v in r.items()}
         print(json.dumps({"forms_progress": "done"}), file=sys.stderr, flush=True)
+    adj = None
+    if not ns.no_adjoint:
+        adj = adjoint_ratio(ns.adjoint_large)
+        print(json.dumps({"adjoint_progress": "done"}), file=sys.stderr, flush=True)
     cpu = None
     if not ns.no_cpu:
         cpu = cpu_baseline(state)
@@ -192,7 +296,7 @@ def main(argv=None) -> int:
             "bytes_per_update": head["bytes_per_update"],
         },
         "effective_frac_hbm": round(head["frac_hbm"], 4),
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adj,
         "forms": forms,
         "cpu_baseline": cpu,
     }

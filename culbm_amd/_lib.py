@@ -61,6 +61,35 @@ class Info(ctypes.Structure):
     ]
 
 
+class AdjointSpec(ctypes.Structure):
+    _fields_ = [
+        ("nsteps", ctypes.c_int32),
+        ("t_begin", ctypes.c_int32), ("t_end", ctypes.c_int32),
+        ("checkpoint_every", ctypes.c_int32),
+        ("want_f0", ctypes.c_int32),
+        ("mem_cap", ctypes.c_int64),
+        ("timing", ctypes.c_int32),
+        ("max_slots", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [
+        ("objective", ctypes.c_double),
+        ("grad_inv_tau", ctypes.c_double),
+        ("forward_steps", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("slots", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("ms_sweep", ctypes.c_float),
+        ("ms_recompute", ctypes.c_float),
+        ("ms_reverse", ctypes.c_float),
+        ("ms_total", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 6),
+    ]
+
+
 _F32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _U32 = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _U16 = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
@@ -86,6 +115,9 @@ SIGNATURES = {
     "culbm_destroy": (ctypes.c_int, [_VP]),
     "culbm_last_error": (ctypes.c_char_p, [_VP]),
     "culbm_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "culbm_adjoint": (ctypes.c_int, [_VP, ctypes.POINTER(AdjointSpec), _VP, ctypes.POINTER(AdjointStats),
+                                     _VP, _VP, _VP, _VP, _VP]),
+    "culbm_adjoint_plan": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I32)]),
     "culbm_probe_cells": (ctypes.c_int, [_I32, _I32, ctypes.c_float, _VP, _VP, _VP, ctypes.c_int64]),
     "culbm_probe_codec": (ctypes.c_int, [_I32, _I32, ctypes.c_float, _F32, _VP, _F32, ctypes.c_int64]),
     "culbm_blocking_axis": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_I32), _VP, _I32]),

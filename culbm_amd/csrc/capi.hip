@@ -13,6 +13,7 @@
 #include "../../include/culbm.h"
 #include "blocking.hpp"
 #include "lbm_kernels.hip"
+#include "adjoint.hip"
 
 using namespace culbm;
 
@@ -982,3 +983,5 @@ int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t i
 }
 
 }  // extern "C"
+
+#include "adjoint_capi.hip"

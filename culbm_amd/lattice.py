@@ -164,6 +164,40 @@ class Lattice:
         self._check(_lib.lib().culbm_get_macros(self._ctx, *[_lib.ptr(a) for a in out]))
         return tuple(out)
 
+    # -- discrete adjoint ----------------------------------------------------
+    def adjoint(self, weight, nsteps: int, window: Tuple[int, int], checkpoint_every: int = 0,
+                mem_cap: int = 0, want_f0: bool = False, timing: bool = False,
+                max_slots: int = 0) -> "AdjointResult":
+        """Objective J = sum_{t in window} sum_x weight(x) u_x(x, t) over the next
+        `nsteps` steps and its gradients (the context ends `nsteps` steps on).
+
+        `window` = (t_begin, t_end), inclusive, in steps from the current state.
+        checkpoint_every = k > 0 keeps every k-th state; 0 picks the binomial
+        schedule that fits `mem_cap` device bytes (0: 90% of free memory) and
+        holds at most `max_slots` states (0: no limit)."""
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        if w.size != self.ncell:
+            raise ValueError(f"weight has {w.size} values, domain has {self.ncell}")
+        spec = _lib.AdjointSpec()
+        spec.nsteps = int(nsteps)
+        spec.t_begin, spec.t_end = int(window[0]), int(window[1])
+        spec.checkpoint_every = int(checkpoint_every)
+        spec.want_f0 = int(bool(want_f0))
+        spec.mem_cap = int(mem_cap)
+        spec.timing = int(bool(timing))
+        spec.max_slots = int(max_slots)
+        st = _lib.AdjointStats()
+        g = [np.empty(self.shape, dtype=np.float64) for _ in range(4)]
+        f0 = np.empty((27,) + self.shape, dtype=np.float32) if want_f0 else None
+        self._check(_lib.lib().culbm_adjoint(self._ctx, ctypes.byref(spec), _lib.ptr(w), ctypes.byref(st),
+                                             *[_lib.ptr(a) for a in g], _lib.ptr(f0)))
+        return AdjointResult(
+            objective=st.objective, grad_rho=g[0], grad_ux=g[1], grad_uy=g[2], grad_uz=g[3],
+            grad_inv_tau=st.grad_inv_tau, grad_f0=f0,
+            stats={"forward_steps": st.forward_steps, "device_bytes": st.device_bytes, "slots": st.slots,
+                   "launches": st.launches, "ms_sweep": st.ms_sweep, "ms_recompute": st.ms_recompute,
+                   "ms_reverse": st.ms_reverse, "ms_total": st.ms_total})
+
     def info(self) -> dict:
         i = _lib.Info()
         self._check(_lib.lib().culbm_get_info(self._ctx, ctypes.byref(i)))
@@ -175,6 +209,28 @@ class Lattice:
             "bytes_per_pop": i.bytes_per_pop,
             "storage_scale": i.storage_scale,
         }
+
+
+@dataclass
+class AdjointResult:
+    """J and its gradients; grad_* are (nz, ny, nx) float64, zero outside EQU cells."""
+    objective: float
+    grad_rho: np.ndarray
+    grad_ux: np.ndarray
+    grad_uy: np.ndarray
+    grad_uz: np.ndarray
+    grad_inv_tau: float
+    grad_f0: Optional[np.ndarray]
+    stats: dict
+
+
+def adjoint_plan(nsteps: int, checkpoint_every: int = 0, slots: int = 1) -> Tuple[int, int]:
+    """(forward steps, state slots) of a checkpoint schedule; host only."""
+    fwd = ctypes.c_int64()
+    used = ctypes.c_int32()
+    _lib.check(_lib.lib().culbm_adjoint_plan(int(nsteps), int(checkpoint_every), int(slots),
+                                             ctypes.byref(fwd), ctypes.byref(used)))
+    return fwd.value, used.value
 
 
 # ---- single-cell and codec probes (device) --------------------------------

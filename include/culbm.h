@@ -171,6 +171,56 @@ const char* culbm_last_error(const culbm_ctx* ctx);
 /* Name of a return code. */
 const char* culbm_error_string(int code);
 
+/* ---- discrete adjoint (no reference counterpart) ----------------------------
+ * Objective over the states f^t reached after t steps from the current state:
+ *   J = sum_{t = t_begin..t_end} sum_x w(x) u_x(x, t),
+ * u_x(x, t) being what the dump kernel reports for cell x after t steps.
+ * Gradients: dJ/d(rho, ux, uy, uz) of the EQU inputs (the inlet profile),
+ * dJ/dinvTau and, on request, dJ/df^0.  The forward trajectory is the plain
+ * pull step in the context's storage format (every policy produces the same
+ * bits); the adjoint runs in FP32 with FP64 accumulators.  Forward states are
+ * kept as checkpoints and recomputed under a device-memory cap. */
+typedef struct culbm_adjoint_spec {
+  int32_t nsteps;            /* N >= 1 forward steps                                   */
+  int32_t t_begin, t_end;    /* objective window, 0 <= t_begin <= t_end <= N          */
+  int32_t checkpoint_every;  /* k > 0: keep f^0, f^k, f^2k, ...; 0: binomial schedule */
+  int32_t want_f0;           /* also return dJ/df^0                                    */
+  int64_t mem_cap;           /* device bytes the call may allocate; 0: 90% of free     */
+  int32_t timing;            /* fill the ms_* fields (device events per phase)         */
+  int32_t max_slots;         /* binomial: at most this many states held (0: no limit)  */
+  int32_t reserved[4];
+} culbm_adjoint_spec;
+
+typedef struct culbm_adjoint_stats {
+  double objective;          /* J                                                      */
+  double grad_inv_tau;       /* dJ/dinvTau                                             */
+  int64_t forward_steps;     /* forward steps run, the first sweep included            */
+  int64_t device_bytes;      /* workspace allocated by the call                        */
+  int32_t slots;             /* states held as checkpoints                             */
+  int32_t launches;          /* kernel launches                                        */
+  float ms_sweep;            /* first forward sweep (with the objective)               */
+  float ms_recompute;        /* forward steps recomputed during the reverse sweep      */
+  float ms_reverse;          /* adjoint steps                                          */
+  float ms_total;            /* whole call, set-up and copies included                 */
+  int32_t reserved[6];
+} culbm_adjoint_stats;
+
+/* Run N steps from the current state (the context ends N steps further on,
+ * as after culbm_step) and the reverse sweep.  weight: nx*ny*nz doubles (the
+ * objective's w).  grad_rho/ux/uy/uz: nx*ny*nz doubles each or NULL, zero
+ * outside EQU cells.  grad_f0: 27*nx*ny*nz floats ([27][nz][ny][nx]) or NULL;
+ * required when spec->want_f0.  Fails with CULBM_ERR_INVALID when the
+ * checkpoints do not fit mem_cap. */
+int culbm_adjoint(culbm_ctx* ctx, const culbm_adjoint_spec* spec, const double* weight,
+                  culbm_adjoint_stats* stats, double* grad_rho, double* grad_ux,
+                  double* grad_uy, double* grad_uz, float* grad_f0);
+
+/* Checkpoint schedule without running it: forward steps and state slots the
+ * reverse sweep of nsteps steps takes with checkpoint_every (> 0) or, for 0,
+ * the binomial schedule with `slots` states.  Host only. */
+int culbm_adjoint_plan(int32_t nsteps, int32_t checkpoint_every, int32_t slots,
+                       int64_t* forward_steps, int32_t* slots_used);
+
 /* ---- test probes (single cells, codecs); device arrays are managed here ---- */
 
 /* op 0: collide f[n][27] in place; op 1: equilibrium of in4[n][4] = (rho,u)

@@ -100,6 +100,7 @@ inline void feq_from(const Eq<R>& e, R* feq) {
   const R rhos = W<R>::WS * e.rho, rhoe = W<R>::WE * e.rho, rhoc = W<R>::WC * e.rho;
   const R rhom1s = W<R>::WS * e.rhom1, rhom1e = W<R>::WE * e.rhom1,
           rhom1c = W<R>::WC * e.rhom1;
+#pragma GCC unroll 27
   for (int i = 0; i < 27; ++i) {
     if (i == 13) continue;
     const int k = wclass(i);
@@ -122,6 +123,7 @@ inline void equ(R rho, R vx, R vy, R vz, R* f) {
 template <typename R>
 inline void moments(const R* f, R& rho, R& vx, R& vy, R& vz) {
   rho = (R(1) + f[0]);
+#pragma GCC unroll 13
   for (int k = 1; k < 27; k += 2) rho = rho + (f[k] + f[k + 1]);
   const R inv = R(1) / rho;
   static const int px[9] = {2, 5, 8, 11, 14, 17, 20, 23, 26};
@@ -133,6 +135,7 @@ inline void moments(const R* f, R& rho, R& vx, R& vy, R& vz) {
   vx = f[px[0]] - f[mx[0]];
   vy = f[py[0]] - f[my[0]];
   vz = f[pz[0]] - f[mz[0]];
+#pragma GCC unroll 8
   for (int k = 1; k < 9; ++k) {
     vx = vx + (f[px[k]] - f[mx[k]]);
     vy = vy + (f[py[k]] - f[my[k]]);
@@ -154,6 +157,7 @@ inline void collide(R inv_tau, R* f) {
   R feq[27];
   feq_from(e, feq);
   const R omt = R(1) - inv_tau;
+#pragma GCC unroll 27
   for (int i = 0; i < 27; ++i) {
     if (i == 13) continue;
     f[i] = inv_tau * feq[i] + omt * f[i];
@@ -637,6 +641,203 @@ double orc_adjoint_f64(int nx, int ny, int nz, const uint32_t* flag, const doubl
     std::swap(lam, nxt);  // lambda^t
   }
   if (grad_f0) std::memcpy(grad_f0, lam.data(), sizeof(double) * 27 * n);
+  if (grad_tau) *grad_tau = gt;
+  return J;
+}
+
+}  // extern "C"
+
+// ---- FP64 forward objective and checkpointed, parallel FP64 adjoint --------
+//
+// The same model as orc_adjoint_f64 for grids where keeping every state does
+// not fit: states are kept every `every` steps and each segment is recomputed
+// once.  Per-cell work runs in parallel; the transposed pull runs in parallel
+// over the 27 directions (each direction writes its own plane) and every sum
+// keeps the cell order of orc_adjoint_f64, so both return the same bits.
+
+namespace {
+
+struct AdjIn {
+  Dom d;
+  const uint32_t* flag;
+  const double *rho, *ux, *uy, *uz;
+  double inv_tau;
+  const double* weight;
+};
+
+// Offsets c - src of the unclamped pull, per direction (interior cells).
+inline void pull_offsets(const Dom& d, int64_t* off) {
+  for (int i = 0; i < 27; ++i)
+    off[i] = int64_t(i) * d.n - (cx_of(i) + int64_t(cy_of(i)) * d.nx + int64_t(cz_of(i)) * d.nx * d.ny);
+}
+
+// One FP64 pull step with double EQU inputs; cells without STORE keep dst.
+void step_f64_in(const AdjIn& in, const double* src, double* dst, int nth) {
+  const Dom& d = in.d;
+  int64_t off[27];
+  pull_offsets(d, off);
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nth)
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const int64_t c = (int64_t(z) * d.ny + y) * d.nx + x;
+        const uint32_t fl = in.flag[c];
+        if (!(fl & F_STORE)) continue;
+        double f[27] = {0};
+        const bool inner = x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
+        if ((fl & F_LOAD) && inner)
+          for (int i = 0; i < 27; ++i) f[i] = src[c + off[i]];
+        else if (fl & F_LOAD)
+          pull_cell(d, src, x, y, z, f);
+        if (fl & F_EQU) equ<double>(in.rho[c], in.ux[c], in.uy[c], in.uz[c], f);
+        if (fl & F_BB) bounce_back(f);
+        if (fl & F_COLLIDE) collide<double>(in.inv_tau, f);
+        for (int i = 0; i < 27; ++i) dst[int64_t(i) * d.n + c] = f[i];
+      }
+}
+
+// sum_x w(x) u_x(g(x)) in cell order; with mu: mu += w du_x/dg.
+double objective_f64(const AdjIn& in, const double* f, double* mu, std::vector<double>& part, int nth) {
+  const Dom& d = in.d;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nth)
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const int64_t c = (int64_t(z) * d.ny + y) * d.nx + x;
+        part[size_t(c)] = 0.0;
+        if (in.weight[c] == 0.0) continue;
+        double g[27], r, vx, vy, vz;
+        pull_cell(d, f, x, y, z, g);
+        moments(g, r, vx, vy, vz);
+        part[size_t(c)] = in.weight[c] * vx;
+        if (mu)
+          for (int i = 0; i < 27; ++i) mu[int64_t(i) * d.n + c] += in.weight[c] * (cx_of(i) - vx) / r;
+      }
+  double s = 0.0;
+  for (int64_t c = 0; c < d.n; ++c)
+    if (in.weight[c] != 0.0) s += part[size_t(c)];
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 forward of nsteps steps from f0 (EQU inputs as double) and
+// J = sum_{t in [t0, t1]} sum_x w(x) u_x(g^t(x)).  f_end (27n, may be NULL)
+// receives f^nsteps.
+double orc_objective_f64(int nx, int ny, int nz, const uint32_t* flag, const double* rho_in,
+                         const double* ux_in, const double* uy_in, const double* uz_in,
+                         double inv_tau, int nsteps, int t0, int t1, const double* weight,
+                         const double* f0, double* f_end, int nthreads) {
+  const AdjIn in{{nx, ny, nz, int64_t(nx) * ny * nz}, flag, rho_in, ux_in, uy_in, uz_in, inv_tau, weight};
+  const int64_t n = in.d.n;
+  std::vector<double> a(f0, f0 + 27 * n), b(a), part(static_cast<size_t>(n));
+  double J = 0.0;
+  for (int t = 0;; ++t) {
+    if (t >= t0 && t <= t1) J += objective_f64(in, a.data(), nullptr, part, nthreads);
+    if (t == nsteps) break;
+    step_f64_in(in, a.data(), b.data(), nthreads);
+    std::swap(a, b);
+  }
+  if (f_end) std::memcpy(f_end, a.data(), sizeof(double) * 27 * n);
+  return J;
+}
+
+// orc_adjoint_f64 with states kept every `every` steps (every >= 1) and the
+// work spread over `nthreads` threads.  Same outputs, same bits.
+double orc_adjoint_ckpt_f64(int nx, int ny, int nz, const uint32_t* flag, const double* rho_in,
+                            const double* ux_in, const double* uy_in, const double* uz_in,
+                            double inv_tau, int nsteps, int t0, int t1, const double* weight,
+                            const double* f0, int every, double* grad_rho, double* grad_ux,
+                            double* grad_uy, double* grad_uz, double* grad_tau, double* grad_f0,
+                            int nthreads) {
+  const AdjIn in{{nx, ny, nz, int64_t(nx) * ny * nz}, flag, rho_in, ux_in, uy_in, uz_in, inv_tau, weight};
+  const Dom& d = in.d;
+  const int64_t n = d.n;
+  const size_t S = size_t(27 * n);
+  if (every < 1) every = 1;
+  const int nck = (nsteps + every - 1) / every;
+  // checkpoints f^{j*every}, j < nck; f^N kept for the objective at N
+  std::vector<std::vector<double>> ck(static_cast<size_t>(nck));
+  std::vector<double> cur(f0, f0 + S), nxt(cur), part(static_cast<size_t>(n));
+  double J = 0.0;
+  for (int t = 0;; ++t) {
+    if (t % every == 0 && t < nsteps) ck[size_t(t / every)] = cur;
+    if (t >= t0 && t <= t1) J += objective_f64(in, cur.data(), nullptr, part, nthreads);
+    if (t == nsteps) break;
+    step_f64_in(in, cur.data(), nxt.data(), nthreads);
+    std::swap(cur, nxt);
+  }
+  std::vector<double> mu(S, 0.0), lam(S, 0.0), lnext(S, 0.0), dtau(static_cast<size_t>(n), 0.0);
+  for (int64_t c = 0; c < n; ++c) grad_rho[c] = grad_ux[c] = grad_uy[c] = grad_uz[c] = 0.0;
+  auto transpose_pull = [&](std::vector<double>& out) {  // out += P^T mu, cell order per direction
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int i = 0; i < 27; ++i)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x)
+            out[size_t(int64_t(i) * n + pull_src(d, x, y, z, i))] +=
+                mu[size_t(int64_t(i) * n + (int64_t(z) * ny + y) * nx + x)];
+  };
+  if (nsteps >= t0 && nsteps <= t1) objective_f64(in, cur.data(), mu.data(), part, nthreads);
+  transpose_pull(lam);
+  double gt = 0.0;
+  std::vector<std::vector<double>> seg(static_cast<size_t>(every));
+  for (int j = nck - 1; j >= 0; --j) {
+    const int a = j * every, b = std::min(nsteps, (j + 1) * every);
+    seg[0] = ck[size_t(j)];
+    for (int t = a + 1; t < b; ++t) {
+      seg[size_t(t - a)].resize(S);
+      std::copy(seg[size_t(t - a - 1)].begin(), seg[size_t(t - a - 1)].end(), seg[size_t(t - a)].begin());
+      step_f64_in(in, seg[size_t(t - a - 1)].data(), seg[size_t(t - a)].data(), nthreads);
+    }
+    for (int t = b - 1; t >= a; --t) {
+      const double* src = seg[size_t(t - a)].data();
+      std::fill(mu.begin(), mu.end(), 0.0);
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const int64_t c = (int64_t(z) * ny + y) * nx + x;
+            const uint32_t fl = flag[c];
+            dtau[size_t(c)] = 0.0;
+            if (!(fl & F_STORE)) continue;
+            double g[27] = {0}, h[27], lo[27], lh[27];
+            if (fl & F_LOAD) pull_cell(d, src, x, y, z, g);
+            for (int i = 0; i < 27; ++i) h[i] = g[i];
+            if (fl & F_EQU) equ<double>(rho_in[c], ux_in[c], uy_in[c], uz_in[c], h);
+            if (fl & F_BB) bounce_back(h);
+            for (int i = 0; i < 27; ++i) lo[i] = lam[size_t(int64_t(i) * n + c)];
+            if (fl & F_COLLIDE) {
+              dtau[size_t(c)] = adj_collide_f64(inv_tau, h, lo, lh);
+            } else {
+              for (int i = 0; i < 27; ++i) lh[i] = lo[i];
+            }
+            if (fl & F_BB) bounce_back(lh);
+            if (fl & F_EQU) {
+              double g4[4] = {0, 0, 0, 0};
+              adj_feq_inputs_f64(rho_in[c], ux_in[c], uy_in[c], uz_in[c], lh, g4);
+              grad_rho[c] += g4[0];
+              grad_ux[c] += g4[1];
+              grad_uy[c] += g4[2];
+              grad_uz[c] += g4[3];
+            } else if (fl & F_LOAD) {
+              for (int i = 0; i < 27; ++i) mu[size_t(int64_t(i) * n + c)] = lh[i];
+            }
+          }
+      for (int64_t c = 0; c < n; ++c)
+        if ((flag[c] & F_STORE) && (flag[c] & F_COLLIDE)) gt += dtau[size_t(c)];
+      if (t >= t0 && t <= t1) objective_f64(in, src, mu.data(), part, nthreads);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+      for (int64_t c = 0; c < n; ++c)
+        for (int i = 0; i < 27; ++i)
+          lnext[size_t(int64_t(i) * n + c)] = (flag[c] & F_STORE) ? 0.0 : lam[size_t(int64_t(i) * n + c)];
+      transpose_pull(lnext);
+      std::swap(lam, lnext);
+    }
+  }
+  if (grad_f0) std::memcpy(grad_f0, lam.data(), sizeof(double) * S);
   if (grad_tau) *grad_tau = gt;
   return J;
 }

@@ -258,3 +258,49 @@ def adjoint_f64(state, inv_tau, nsteps, t0, t1, weight, f0, rho=None, ux=None, u
     g["inv_tau"] = gt.value
     g["f0"] = gf0.reshape(27, nz, ny, nx)
     return J, g
+
+
+def _adj_fields(state, rho, ux, uy, uz):
+    return [np.ascontiguousarray(v if v is not None else getattr(state, k), np.float64).reshape(-1)
+            for v, k in ((rho, "rho"), (ux, "vx"), (uy, "vy"), (uz, "vz"))]
+
+
+def objective_f64(state, inv_tau, nsteps, t0, t1, weight, f0, rho=None, ux=None, uy=None, uz=None,
+                  nthreads=None, f_end=False):
+    """FP64 forward run of nsteps steps from f0 and J = sum_{t in [t0, t1]}
+    sum_x w(x) u_x(x, t) (EQU inputs as float64, so they can be perturbed).
+    With f_end=True returns (J, f^nsteps)."""
+    fn = lib().orc_objective_f64
+    fn.restype = ctypes.c_double
+    fn.argtypes = [_i, _i, _i, _U32, _F64P, _F64P, _F64P, _F64P, ctypes.c_double, _i, _i, _i,
+                   _F64P, _F64P, ctypes.c_void_p, _i]
+    nz, ny, nx = state.flag.shape
+    out = np.empty(27 * nx * ny * nz) if f_end else None
+    J = fn(nx, ny, nz, np.ascontiguousarray(state.flag, np.uint32).reshape(-1),
+           *_adj_fields(state, rho, ux, uy, uz), float(inv_tau), int(nsteps), int(t0), int(t1),
+           np.ascontiguousarray(weight, np.float64).reshape(-1), np.ascontiguousarray(f0, np.float64).reshape(-1),
+           None if out is None else out.ctypes.data_as(ctypes.c_void_p), threads(nthreads))
+    return (J, out.reshape(27, nz, ny, nx)) if f_end else J
+
+
+def adjoint_ckpt_f64(state, inv_tau, nsteps, t0, t1, weight, f0, every, rho=None, ux=None, uy=None,
+                     uz=None, nthreads=None):
+    """adjoint_f64 with states kept every `every` steps, on `nthreads` threads
+    (same results, bit for bit)."""
+    fn = lib().orc_adjoint_ckpt_f64
+    fn.restype = ctypes.c_double
+    fn.argtypes = [_i, _i, _i, _U32, _F64P, _F64P, _F64P, _F64P, ctypes.c_double, _i, _i, _i,
+                   _F64P, _F64P, _i, _F64P, _F64P, _F64P, _F64P, ctypes.POINTER(ctypes.c_double), _F64P, _i]
+    nz, ny, nx = state.flag.shape
+    n = nx * ny * nz
+    g = {k: np.zeros(n) for k in ("rho", "ux", "uy", "uz")}
+    gf0 = np.zeros(27 * n)
+    gt = ctypes.c_double()
+    J = fn(nx, ny, nz, np.ascontiguousarray(state.flag, np.uint32).reshape(-1),
+           *_adj_fields(state, rho, ux, uy, uz), float(inv_tau), int(nsteps), int(t0), int(t1),
+           np.ascontiguousarray(weight, np.float64).reshape(-1), np.ascontiguousarray(f0, np.float64).reshape(-1),
+           int(every), g["rho"], g["ux"], g["uy"], g["uz"], ctypes.byref(gt), gf0, threads(nthreads))
+    g = {k: v.reshape(nz, ny, nx) for k, v in g.items()}
+    g["inv_tau"] = gt.value
+    g["f0"] = gf0.reshape(27, nz, ny, nx)
+    return J, g

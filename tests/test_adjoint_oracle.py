@@ -96,3 +96,29 @@ def test_initial_state_gradient(case):
             return J_of(st, w, f)
         want = fd(fun, f0[i, z, y, x], 1e-6)
         assert abs(g["f0"][i, z, y, x] - want) <= 1e-6 * max(abs(want), 1e-8) + 1e-12, (i, z, y, x)
+
+
+@pytest.mark.parametrize("every,nth", [(1, 1), (2, 3), (3, 8), (NSTEPS, 4), (100, 2)])
+def test_checkpointed_parallel_adjoint_is_bit_identical(case, every, nth):
+    st, w, f0, J, g = case
+    J2, g2 = po.adjoint_ckpt_f64(st, INV_TAU, NSTEPS, T0, T1, w, f0, every, nthreads=nth)
+    assert J2 == J
+    assert g2["inv_tau"] == g["inv_tau"]
+    for k in ("rho", "ux", "uy", "uz", "f0"):
+        assert np.array_equal(g2[k], g[k]), k
+
+
+def test_forward_objective_matches_the_adjoint_run(case):
+    st, w, f0, J, g = case
+    for nth in (1, 5):
+        assert po.objective_f64(st, INV_TAU, NSTEPS, T0, T1, w, f0, nthreads=nth) == J
+
+
+def test_forward_objective_steps_like_the_fp64_simulator(case):
+    # the objective's forward run is the oracle's FP64 step
+    st, w, f0, J, g = case
+    _, f_end = po.objective_f64(st, INV_TAU, NSTEPS, T0, T1, w, f0, f_end=True)
+    sim = po.Sim(st, INV_TAU, np.float64)
+    sim.set_ddf(f0)
+    sim.step(NSTEPS)
+    assert np.array_equal(f_end, sim.ddf())
