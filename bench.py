@@ -188,6 +188,41 @@ def adjoint_ratio(large=False):
     return out
 
 
+def precision_report(n=512, steps=500):
+    """The precision workload at n^3: FP32, FP16-shifted and BF16-shifted storage
+    (in-place + dynamic blocking, innerLoop 5), MLUPS of each and the rho/u
+    deviation of the 16-bit runs from the FP32 run after `steps` steps (fluid
+    cells; u relative to u_in 0.05).  Errors against FP64 are measured at 128^3
+    by tests/test_precision.py."""
+    from culbm_amd import Lattice, LatticeConfig, cases
+
+    st = cases.config_precision(n, seed=3)
+    fluid = (st.flag & cases.DUMP) != 0
+    out, ref = {}, None
+    for storage in ("fp32", "fp16", "bf16"):
+        lat = Lattice(LatticeConfig(domain=st.domain, inv_tau=INV_TAU, stream_policy=1, opt_policy=3,
+                                    inner_loop=5, storage=storage))
+        lat.set_state(st.flag, st.rho, st.vx, st.vy, st.vz)
+        lat.init_ddf("feq")
+        ms = lat.time_steps(steps)
+        lat.dump()
+        rho, vx, vy, vz = lat.macros()
+        lat.close()
+        r = {"mlups": round(st.flag.size * steps / (ms * 1e-3) / 1e6, 1)}
+        if ref is None:
+            ref = (rho, vx, vy, vz)
+        else:
+            d_rho = np.abs(rho.astype(np.float64) - ref[0])[fluid]
+            d_u = np.sqrt(sum((a.astype(np.float64) - b) ** 2 for a, b in zip((vx, vy, vz), ref[1:])))[fluid] / 0.05
+            r.update({"rho_max_vs_fp32": float(d_rho.max()), "rho_rms_vs_fp32": float(np.sqrt((d_rho ** 2).mean())),
+                      "u_max_vs_fp32": float(d_u.max()), "u_rms_vs_fp32": float(np.sqrt((d_u ** 2).mean()))})
+        out[storage] = r
+        print(json.dumps({"precision_progress": storage}), file=sys.stderr, flush=True)
+    out["fp16_over_fp32_mlups"] = round(out["fp16"]["mlups"] / out["fp32"]["mlups"], 3)
+    out["workload"] = f"{n}^3 zero-gradient channel, rho = 1 + U(-1e-3, 1e-3), u_in 0.05, invTau 0.5, {steps} steps"
+    return out
+
+
 def cpu_baseline(state, budget_s=15.0):
     """FP32 OpenMP restatement of the plain pull step on the same workload."""
     from oracle import pyoracle
@@ -223,6 +258,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="measure only the headline form")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
+    ap.add_argument("--precision", action="store_true",
+                    help="also run the 512^3 precision workload in the three storage formats")
     ap.add_argument("--adjoint-large", action="store_true",
                     help="also run the 256^3 FP16 adjoint with binomial checkpoints")
     ns = ap.parse_args(argv)
@@ -247,6 +284,7 @@ def main(argv=None) -> int:
     if not ns.no_adjoint:
         adj = adjoint_ratio(ns.adjoint_large)
         print(json.dumps({"adjoint_progress": "done"}), file=sys.stderr, flush=True)
+    prec = precision_report() if ns.precision else None
     cpu = None
     if not ns.no_cpu:
         cpu = cpu_baseline(state)
@@ -297,6 +335,7 @@ def main(argv=None) -> int:
         },
         "effective_frac_hbm": round(head["frac_hbm"], 4),
         "adjoint_forward_ratio": adj,
+        "precision": prec,
         "forms": forms,
         "cpu_baseline": cpu,
     }

@@ -232,6 +232,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint32_t> aux(size_t(n), 0u), equ_cells, obj_idx;
   std::vector<double> obj_w;
+  std::vector<float> wobj(size_t(n), 0.0f);
   int64_t n_keep = 0;
   for (int64_t i = 0; i < n; ++i) {
     const uint8_t f = fl[size_t(i)];
@@ -246,6 +247,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
     if (w != 0.0) {
       obj_idx.push_back(uint32_t(i));
       obj_w.push_back(w);
+      wobj[size_t(i)] = float(w);
     }
   }
   const int64_t n_equ = int64_t(equ_cells.size());
@@ -256,7 +258,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   const uint32_t obj_blk = uint32_t(std::max<size_t>((obj_idx.size() + kBlock - 1) / kBlock, 1));
   const size_t fixed = 2 * mu_b + size_t(n) * 4 + size_t(4 * n_equ) * 8 +
                        size_t(27 * n_keep) * 4 + size_t(a.nblk) * 8 + size_t(obj_blk) * 8 +
-                       obj_idx.size() * 12 + 2 * state + 16 * 256;
+                       obj_idx.size() * 12 + 2 * state + size_t(n) * 4 + 20 * 256;
   int64_t cap = sp.mem_cap;
   if (cap == 0) {
     size_t fr = 0, tot = 0;
@@ -290,7 +292,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   float* mu[2] = {nullptr, nullptr};
   uint32_t *d_aux = nullptr, *d_idx = nullptr;
   double *d_grad4 = nullptr, *d_tau = nullptr, *d_j = nullptr, *d_w = nullptr;
-  float* d_carry = nullptr;
+  float *d_carry = nullptr, *d_wobj = nullptr;
   int rc = CULBM_OK;
   for (auto& s : slot)
     if ((rc = mem.get(&s, state))) return rc;
@@ -303,6 +305,8 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       (rc = mem.get(&d_idx, obj_idx.size() * 4)) || (rc = mem.get(&d_w, obj_w.size() * 8)))
     return rc;
   if (sp.want_f0 && (rc = mem.get(&d_carry, size_t(27 * n_keep) * 4))) return rc;
+  if ((rc = mem.get(&d_wobj, size_t(n) * 4))) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(d_wobj, wobj.data(), size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream));
 
   HIPCHK(ctx, hipMemcpyAsync(d_aux, aux.data(), size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream));
   if (!obj_idx.empty()) {
@@ -408,9 +412,9 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       j.a.src = src;
       j.mu_in = mu[mc];
       j.mu_out = mu[mc ^ 1];
+      j.wobj = in_window(o.t) ? d_wobj : nullptr;
       hipLaunchKernelGGL((k_adjoint<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, j);
       ++launches;
-      if (in_window(o.t)) objective(src, mu[mc ^ 1]);
       mc ^= 1;
       if ((rc = timer.mark(2))) return rc;
     }

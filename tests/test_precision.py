@@ -1,0 +1,75 @@
+"""Storage precision: FP32, FP16-shifted and BF16-shifted populations (collision
+in FP32), in-place streaming with dynamic temporal blocking, against the FP64
+oracle on the precision workload's generator (zero-gradient outlet channel,
+u_in 0.05, rho = 1 + U(-1e-3, 1e-3) from the seed, u = 0, populations at feq),
+scaled to 128^3.
+
+Errors are over the fluid (DUMP) cells after the run: max and RMS of |d rho|
+(rho ~ 1, so absolute = relative) and of |d u| / u_in, and ||d u|| / ||u|| (L2).  The default run takes
+100 steps; CULBM_PRECISION_FULL=1 runs the workload's 500 steps."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from culbm_amd import Lattice, LatticeConfig, cases
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+U_IN = 0.05
+INV_TAU = 0.5
+STEPS = 500 if os.environ.get("CULBM_PRECISION_FULL") else 100
+# bars on (max |d rho|, max |d u| / u_in).  FP32: 1e-5.  The 16-bit formats add one
+# rounding per population per step (2^-11 relative for FP16, 2^-8 for BF16), and
+# the velocity error grows about linearly with the steps: FP16 stays within 1e-3
+# for 100 steps and near 5e-3 after 500 (max over cells; RMS 1.6e-3).
+BARS = {100: {"fp32": (1e-5, 1e-5), "fp16": (1e-3, 1e-3), "bf16": (4e-3, 4e-2)},
+        500: {"fp32": (1e-5, 1e-5), "fp16": (1e-3, 1e-2), "bf16": (1e-2, 1e-1)}}[STEPS]
+
+
+@pytest.fixture(scope="module")
+def state():
+    return cases.config_precision(N, seed=3)
+
+
+@pytest.fixture(scope="module")
+def fp64(state):
+    from oracle import pyoracle as po
+
+    sim = po.Sim(state, INV_TAU, np.float64)
+    sim.init_feq()
+    sim.step(STEPS)
+    return sim.dump()
+
+
+def errors(got, ref, fluid):
+    d_rho = np.abs(got[0].astype(np.float64) - ref[0])[fluid]
+    d_u = np.sqrt(sum((g.astype(np.float64) - r) ** 2 for g, r in zip(got[1:], ref[1:])))[fluid] / U_IN
+    u_ref = np.sqrt(sum(r ** 2 for r in ref[1:]))[fluid] / U_IN
+    return {"rho_max": float(d_rho.max()), "rho_rms": float(np.sqrt((d_rho ** 2).mean())),
+            "u_max": float(d_u.max()), "u_rms": float(np.sqrt((d_u ** 2).mean())),
+            "u_rel_l2": float(np.sqrt((d_u ** 2).sum() / (u_ref ** 2).sum()))}
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_storage_error_against_fp64(state, fp64, storage):
+    cfg = LatticeConfig(domain=state.domain, inv_tau=INV_TAU, stream_policy=1, opt_policy=3,
+                        inner_loop=5, storage=storage)
+    with Lattice(cfg) as lat:
+        lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
+        lat.init_ddf("feq")
+        lat.step(STEPS)
+        lat.dump()
+        got = lat.macros()
+    fluid = (state.flag & cases.DUMP) != 0
+    e = errors(got, fp64, fluid)
+    print(f"{storage} {N}^3 {STEPS} steps vs FP64: " + ", ".join(f"{k} {v:.3e}" for k, v in e.items()),
+          flush=True)
+    bar_rho, bar_u = BARS[storage]
+    assert e["rho_max"] <= bar_rho
+    assert e["u_max"] <= bar_u
+    # the flow has developed
+    assert np.abs(fp64[1][fluid]).max() > 0.2 * U_IN
