@@ -493,16 +493,27 @@ int alloc(culbm_ctx* ctx, void** p, size_t bytes) {
   return CULBM_OK;
 }
 
-// Planes per wavefront chunk: the chunk's working set (D + T + 1 planes of
-// every buffer plus flags) within ~160 MB of the 256 MB Infinity Cache.
+// Planes per wavefront chunk, from two limits:
+//  * fit: the working set (D + T + 1 planes of every buffer plus flags) within
+//    ~160 MB of the 256 MB Infinity Cache;
+//  * floor: enough cells per launch that ramp-up and tail stay small.
+// Measured on 288x272 planes (innerLoop 5): FP32 runs best at 32 planes (2.5M
+// cells per launch, 8 % above the cache-fitting 13), 16-bit in-place at 16
+// planes (1.25M cells); at 128^3 the cache-fitting 83 planes beat a single
+// chunk of the whole domain by 16 %, so the floor never takes the depth past
+// half the domain.
 int32_t auto_depth(const culbm_ctx* ctx) {
-  const double plane = double(ctx->cfg.nx) * ctx->cfg.ny *
-                       (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
+  const double cells = double(ctx->cfg.nx) * ctx->cfg.ny;
+  const double plane = cells * (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
   const double budget = 160.0 * 1024 * 1024;
   const int32_t T = std::max(1, ctx->cfg.inner_loop);
-  int32_t d = int32_t(budget / plane) - T - 1;
-  d = std::max(d, 2);
-  return std::min(d, ctx->cfg.nz);
+  const int32_t nz = ctx->cfg.nz;
+  const int32_t fit = std::max(int32_t(budget / plane) - T - 1, 2);
+  const bool narrow = ctx->inplace && ctx->bpp == 2;
+  const int32_t floor_ = int32_t(std::ceil((narrow ? 1.25e6 : 2.5e6) / cells));
+  const int32_t d = narrow ? std::min(floor_, std::max(fit, nz / 2))
+                           : std::min(std::max(fit, floor_), std::max(fit, nz / 2));
+  return std::max(2, std::min(d, nz));
 }
 
 }  // namespace
