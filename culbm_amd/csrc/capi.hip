@@ -843,6 +843,40 @@ int culbm_set_ddf(culbm_ctx* ctx, const float* ddf) {
   return rc;
 }
 
+int culbm_moment_sums(culbm_ctx* ctx, uint32_t flag_mask, double* sums) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (!sums) return fail(ctx, CULBM_ERR_INVALID, "sums is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  StepArgs a = full_args(ctx);
+  double* part = nullptr;
+  HIPCHK(ctx, hipMalloc(&part, size_t(a.nblk) * 4 * sizeof(double)));
+  const void* buf = ctx->inplace ? ctx->buf[0] : ctx->buf[ctx->cur];
+  int rc = by_storage(ctx->cfg.storage, [&](auto s) {
+    using S = decltype(s);
+    if (!ctx->inplace)
+      hipLaunchKernelGGL((k_moment_sums<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
+                         ctx->flag, flag_mask, part);
+    else if (ctx->parity == 1)  // last stores used the E rules
+      hipLaunchKernelGGL((k_moment_sums<S, 2>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
+                         ctx->flag, flag_mask, part);
+    else
+      hipLaunchKernelGGL((k_moment_sums<S, 1>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
+                         ctx->flag, flag_mask, part);
+    return hipGetLastError() == hipSuccess ? CULBM_OK : CULBM_ERR_HIP;
+  });
+  std::vector<double> h(size_t(a.nblk) * 4);
+  if (!rc && hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream) != hipSuccess)
+    rc = CULBM_ERR_HIP;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(part);
+  if (rc) return fail(ctx, rc, "moment-sum kernel failed");
+  for (int q = 0; q < 4; ++q) sums[q] = 0.0;
+  for (uint32_t b = 0; b < a.nblk; ++b)
+    for (int q = 0; q < 4; ++q) sums[q] += h[4 * b + q];
+  return CULBM_OK;
+}
+
 int culbm_get_info(culbm_ctx* ctx, culbm_info* info) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   if (!info) return fail(ctx, CULBM_ERR_INVALID, "info is NULL");

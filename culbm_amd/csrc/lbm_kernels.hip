@@ -544,6 +544,68 @@ __global__ __launch_bounds__(kBlock) void k_twist_to_plain(const StepArgs a, con
   }
 }
 
+// Stored populations of cell k (the post-collision view of culbm_get_ddf):
+// MODE 0 pull layout, MODE 1 / 2 in-place layout left by a step of parity 1 / 0.
+template <class S, int MODE>
+__device__ __forceinline__ void stored_cell(const StepArgs& a, const Cell& k, const void* buf, float* f) {
+  const typename S::T* b = static_cast<const typename S::T*>(buf);
+  const int32_t sy = a.px, sz = a.px * a.py;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    if constexpr (MODE == 0) {
+      f[i] = S::dec(b[i * a.dstride + k.l], a.inv_scale);
+    } else {
+      constexpr int P = MODE == 1 ? 1 : 0;
+      const int slot = P == 1 ? 26 - i : i;
+      const int32_t o = (cx_of(i) == 1 ? 1 : 0) + (cy_of(i) == 1 ? sy : 0) + (cz_of(i) == 1 ? sz : 0);
+      f[i] = S::dec(b[slot * a.dstride + (k.l + uint32_t(o))], a.inv_scale);
+    }
+  }
+}
+
+// FP64 sums of rho = 1 + sum f and of the momentum sum c f over the cells whose
+// flag shares a bit with `mask` (every cell for mask 0); one partial sum of
+// each per workgroup, added in a fixed order by the host.
+template <class S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_moment_sums(const StepArgs a, const void* buf,
+                                                        const uint32_t* flag, uint32_t mask,
+                                                        double* part) {
+  const Cell k = locate(a, blockIdx.x);
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k.ok && (mask == 0u || (flag[k.c] & mask))) {
+    float f[27];
+    stored_cell<S, MODE>(a, k, buf, f);
+    double rho = 1.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      const double v = f[i];
+      rho += v;
+      jx += cx_of(i) * v;
+      jy += cy_of(i) * v;
+      jz += cz_of(i) * v;
+    }
+    m[0] = rho;
+    m[1] = jx;
+    m[2] = jy;
+    m[3] = jz;
+  }
+  __shared__ double red[4][kBlock / 64];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double v = m[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
+    part[4 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
 // Equilibrium of the rho/u fields as post-collision state (plain FP32).
 __global__ __launch_bounds__(kBlock) void k_feq_plain(const StepArgs a, float* plain) {
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));

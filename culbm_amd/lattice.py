@@ -164,6 +164,14 @@ class Lattice:
         self._check(_lib.lib().culbm_get_macros(self._ctx, *[_lib.ptr(a) for a in out]))
         return tuple(out)
 
+    def moment_sums(self, flag_mask: int = 0) -> Tuple[float, float, float, float]:
+        """FP64 (sum rho, sum rho u_x, sum rho u_y, sum rho u_z) of the stored
+        populations over the cells whose flag shares a bit with flag_mask
+        (every cell for 0): the mass-conservation check."""
+        out = (ctypes.c_double * 4)()
+        self._check(_lib.lib().culbm_moment_sums(self._ctx, int(flag_mask), out))
+        return tuple(out)
+
     # -- discrete adjoint ----------------------------------------------------
     def adjoint(self, weight, nsteps: int, window: Tuple[int, int], checkpoint_every: int = 0,
                 mem_cap: int = 0, want_f0: bool = False, timing: bool = False,

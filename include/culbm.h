@@ -159,6 +159,12 @@ int culbm_get_ddf(culbm_ctx* ctx, float* ddf);
  * storage format).  The step counter is kept. */
 int culbm_set_ddf(culbm_ctx* ctx, const float* ddf);
 
+/* FP64 reductions of the stored populations (the culbm_get_ddf view):
+ * sums[0] = sum of rho = 1 + sum_i f_i, sums[1..3] = sum of the momentum
+ * sum_i c_i f_i, over the cells whose flag shares a bit with flag_mask (every
+ * cell for 0).  The mass-conservation check; no reference counterpart. */
+int culbm_moment_sums(culbm_ctx* ctx, uint32_t flag_mask, double* sums);
+
 int culbm_get_info(culbm_ctx* ctx, culbm_info* info);
 
 /* Release device memory and the stream.  NULL is accepted. */
