@@ -72,6 +72,13 @@ def run_form(state, name, steps, warmup, inner_loop):
     wall = time.perf_counter() - t0
     info = lat.info()
     prof = lat.profile_steps(min(steps, 50))  # per-launch events: kernel time only
+    # the reference's measurement: device-timed windows, one warm-up window, then the median
+    lat.run_window(20)
+    ncell_w = DOMAIN[0] * DOMAIN[1] * DOMAIN[2]
+    windows = []
+    for _ in range(5):
+        done, ms = lat.run_window(20)
+        windows.append(ncell_w * done / (ms * 1e-3) / 1e6)
     lat.close()
     ncell = DOMAIN[0] * DOMAIN[1] * DOMAIN[2]
     bpu = 27 * 2 * info["bytes_per_pop"]  # algorithmic bytes per cell-update
@@ -88,6 +95,7 @@ def run_form(state, name, steps, warmup, inner_loop):
         "launches_per_step": prof["launches"] / prof["steps"],
         "wave_depth": info["wave_depth"],
         "frac_hbm": mlups * 1e6 * bpu / 1e9 / HBM_PEAK_GBS,
+        "window_median_mlups": float(np.median(windows)),
     }
 
 
@@ -225,6 +233,7 @@ def precision_report(n=512, steps=500):
 
 def cpu_baseline(state, budget_s=15.0):
     """FP32 OpenMP restatement of the plain pull step on the same workload."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime loads
     from oracle import pyoracle
 
     sim = pyoracle.Sim(state, INV_TAU, np.float32)
@@ -242,6 +251,8 @@ def cpu_baseline(state, budget_s=15.0):
         "value": ncell * n / dt / 1e6,
         "unit": "MLUPS",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
         "kind": "port",
         "sample": f"FP32 OpenMP plain-pull restatement, {n} steps of the full 288x272x280 workload "
                   f"after 2 warm-up steps ({threads} threads)",
@@ -334,6 +345,7 @@ def main(argv=None) -> int:
             "bytes_per_update": head["bytes_per_update"],
         },
         "effective_frac_hbm": round(head["frac_hbm"], 4),
+        "window_median_mlups": round(head["window_median_mlups"], 1),
         "adjoint_forward_ratio": adj,
         "precision": prec,
         "forms": forms,
