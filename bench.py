@@ -276,7 +276,8 @@ def main(argv=None) -> int:
     ns = ap.parse_args(argv)
     if ns.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not provided "
-                                   "(the z-slab decomposition over xGMI is described in DESIGN.md)"}))
+                                   "(a z-slab decomposition with one 9-direction face exchange per step over xGMI "
+                                   "is the multi-GPU design)"}))
         return 2
 
     from culbm_amd import cases

@@ -82,7 +82,7 @@ typedef struct culbm_config {
   int32_t stream_policy;    /* --streamPolicy                                */
   int32_t opt_policy;       /* --optPolicy                                   */
   int32_t inner_loop;       /* --innerLoop: steps per temporal-blocking pass */
-  int32_t block_dim[3];     /* --blockDim (tile-shape knob, see DESIGN)      */
+  int32_t block_dim[3];     /* --blockDim (checked; kernels pick their own)  */
   int32_t grid_dim[3];      /* --gridDim                                     */
   int32_t use_ring;         /* --useRingDDFBuf                               */
   int32_t device;           /* --devId                                       */
