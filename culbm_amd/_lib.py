@@ -45,7 +45,8 @@ class Config(ctypes.Structure):
         ("storage", ctypes.c_int32),
         ("storage_scale", ctypes.c_float),
         ("wave_depth", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("rounding", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

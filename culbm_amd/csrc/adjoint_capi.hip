@@ -331,11 +331,20 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   HIPCHK(ctx, hipGetLastError());
 
   int32_t launches = 0;
-  auto step = [&](const void* src, void* dst) {
+  // f^{t_out} from f^{t_out - 1}: the state numbers continue the context's step
+  // counter, so stochastic rounding reproduces the context's own trajectory
+  const uint32_t s0 = uint32_t(ctx->step);
+  auto step = [&](const void* src, void* dst, int32_t t_out) {
     StepArgs s = a;
     s.src = src;
     s.dst = dst;
-    hipLaunchKernelGGL((k_pull<S>), dim3(s.nblk), dim3(kBlock), 0, ctx->stream, s);
+    s.state = s0 + uint32_t(t_out);
+    bool sr = false;
+    if constexpr (S::kBytes == 2) sr = ctx->sr;
+    if (sr)
+      hipLaunchKernelGGL((k_pull<S, true>), dim3(s.nblk), dim3(kBlock), 0, ctx->stream, s);
+    else
+      hipLaunchKernelGGL((k_pull<S, false>), dim3(s.nblk), dim3(kBlock), 0, ctx->stream, s);
     ++launches;
   };
   auto in_window = [&](int32_t t) { return t >= sp.t_begin && t <= sp.t_end; };
@@ -369,7 +378,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       dst = slot[size_t(plan.sweep[next_store++].second)];
     else
       dst = free_work(cur);
-    step(cur, dst);
+    step(cur, dst, t + 1);
     cur = dst;
   }
   if ((rc = timer.mark(0))) return rc;
@@ -402,7 +411,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       const void* p = src;
       for (int32_t s = 0; s < o.steps; ++s) {
         void* dst = (s == o.steps - 1 && o.dst >= 0) ? slot[size_t(o.dst)] : free_work(p);
-        step(p, dst);
+        step(p, dst, o.t + s + 1);
         p = dst;
       }
       if (o.dst < 0) last_work = p;

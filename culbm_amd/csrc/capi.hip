@@ -28,6 +28,7 @@ struct culbm_ctx {
   int64_t dstride = 0;     // population elements per direction
   int32_t bpp = 4;         // bytes per population
   bool inplace = false;
+  bool sr = false;         // stochastic rounding of the 16-bit stores
   void* buf[2] = {nullptr, nullptr};
   int cur = 0;             // pull: index of the buffer holding the current state
   int parity = 1;          // in-place: parity of the next step (1 = O, 0 = E)
@@ -171,6 +172,10 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
     return bad(CULBM_ERR_INVALID, "storage scale must be a positive power of two, got %g",
                double(c->storage_scale));
   if (c->wave_depth < 0) return bad(CULBM_ERR_INVALID, "wave depth must be >= 0");
+  if (c->rounding != CULBM_ROUND_NEAREST && c->rounding != CULBM_ROUND_STOCHASTIC)
+    return bad(CULBM_ERR_INVALID, "rounding must be 0 (nearest) or 1 (stochastic), got %d", c->rounding);
+  if (c->rounding == CULBM_ROUND_STOCHASTIC && c->storage == CULBM_STORAGE_FP32)
+    return bad(CULBM_ERR_INVALID, "stochastic rounding applies to FP16 or BF16 storage");
   if (c->device < 0) return bad(CULBM_ERR_INVALID, "devId must be >= 0");
   if (msg && msg_len) msg[0] = 0;
   return CULBM_OK;
@@ -246,6 +251,19 @@ int prof_end(culbm_ctx* ctx) {
   return CULBM_OK;
 }
 
+// One step over planes [z0, z1); a.state is the number of the state it produces.
+template <class S, bool SR>
+void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity) {
+  if (ctx->inplace) {
+    if (parity)
+      hipLaunchKernelGGL((k_twist<S, 1, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((k_twist<S, 0, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_pull<S, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+  }
+}
+
 template <class S>
 int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) {
   set_range(a, z0, z1);
@@ -254,12 +272,14 @@ int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) 
   if (ctx->inplace) {
     a.src = ctx->buf[0];
     a.dst = ctx->buf[0];
-    if (parity)
-      hipLaunchKernelGGL((k_twist<S, 1>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+  }
+  if constexpr (S::kBytes == 2) {
+    if (ctx->sr)
+      launch_step_kernel<S, true>(ctx, a, parity);
     else
-      hipLaunchKernelGGL((k_twist<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+      launch_step_kernel<S, false>(ctx, a, parity);
   } else {
-    hipLaunchKernelGGL((k_pull<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+    launch_step_kernel<S, false>(ctx, a, parity);
   }
   HIPCHK(ctx, hipGetLastError());
   return prof_end(ctx);
@@ -274,6 +294,7 @@ template <class S>
 int wave_pass(culbm_ctx* ctx, int32_t T) {
   StepArgs a = base_args(ctx);
   const int32_t nz = ctx->cfg.nz, D = ctx->wave_depth;
+  const uint32_t base = uint32_t(ctx->step);
   const int32_t K = (nz + T - 1 + D - 1) / D;
   for (int32_t k = 0; k < K; ++k) {
     for (int32_t t = 1; t <= T; ++t) {
@@ -284,25 +305,29 @@ int wave_pass(culbm_ctx* ctx, int32_t T) {
         a.src = ctx->buf[(ctx->cur + t - 1) & 1];
         a.dst = ctx->buf[(ctx->cur + t) & 1];
       }
+      a.state = base + uint32_t(t);
       const int rc = launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1);
       if (rc) return rc;
     }
   }
   ctx->cur = (ctx->cur + T) & 1;
   ctx->parity = (ctx->parity + T) & 1;
+  ctx->step += T;
   return CULBM_OK;
 }
 
 // Static blocking: the wavefront schedule of a full pass is fixed by the
 // configuration, so it is captured once into a HIP graph and replayed; the
 // device's command processor then issues the chunk steps back to back with
-// no host round trip.  A shorter final pass uses plain launches.
+// no host round trip.  A shorter final pass uses plain launches, and so does
+// every pass with stochastic rounding (each launch carries its step number).
 template <class S>
 int static_pass(culbm_ctx* ctx, int32_t T) {
-  if (T != ctx->cfg.inner_loop) return wave_pass<S>(ctx, T);
+  if (T != ctx->cfg.inner_loop || ctx->sr) return wave_pass<S>(ctx, T);
   const int key = ctx->inplace ? ctx->parity : ctx->cur;
   if (!ctx->pass_graph[key]) {
     const int cur = ctx->cur, parity = ctx->parity;
+    const int64_t step = ctx->step;
     const bool prof = ctx->prof;
     ctx->prof = false;
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -312,6 +337,7 @@ int static_pass(culbm_ctx* ctx, int32_t T) {
     ctx->prof = prof;
     ctx->cur = cur;
     ctx->parity = parity;
+    ctx->step = step;
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -329,6 +355,7 @@ int static_pass(culbm_ctx* ctx, int32_t T) {
   if (int rc = prof_end(ctx)) return rc;
   ctx->cur = (ctx->cur + T) & 1;
   ctx->parity = (ctx->parity + T) & 1;
+  ctx->step += T;
   return CULBM_OK;
 }
 
@@ -346,18 +373,27 @@ int fused_pass(culbm_ctx* ctx, int32_t T) {
   for (int32_t s = 0; s + 1 < T; s += 2) {
     fa.a.src = ctx->buf[ctx->cur];
     fa.a.dst = ctx->buf[ctx->cur ^ 1];
+    fa.a.state = uint32_t(ctx->step) + 1u;  // the launch produces states state and state + 1
     if (int rc = prof_begin(ctx)) return rc;
-    hipLaunchKernelGGL((k_fused2<S>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
+    bool sr = false;
+    if constexpr (S::kBytes == 2) sr = ctx->sr;
+    if (sr)
+      hipLaunchKernelGGL((k_fused2<S, true>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
+    else
+      hipLaunchKernelGGL((k_fused2<S, false>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
     HIPCHK(ctx, hipGetLastError());
     if (int rc = prof_end(ctx)) return rc;
     ctx->cur ^= 1;
+    ctx->step += 2;
   }
   if (T & 1) {
     StepArgs a = base_args(ctx);
     a.src = ctx->buf[ctx->cur];
     a.dst = ctx->buf[ctx->cur ^ 1];
+    a.state = uint32_t(ctx->step) + 1u;
     if (int rc = launch_step<S>(ctx, a, 0, ctx->cfg.nz, 0)) return rc;
     ctx->cur ^= 1;
+    ctx->step += 1;
   }
   return CULBM_OK;
 }
@@ -372,10 +408,12 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
         a.src = ctx->buf[ctx->cur];
         a.dst = ctx->buf[ctx->cur ^ 1];
       }
+      a.state = uint32_t(ctx->step) + 1u;
       const int rc = launch_step<S>(ctx, a, 0, c.nz, ctx->parity);
       if (rc) return rc;
       ctx->cur ^= 1;
       ctx->parity ^= 1;
+      ctx->step += 1;
     }
   } else {
     int32_t left = nsteps;
@@ -388,7 +426,6 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
       left -= T;
     }
   }
-  ctx->step += nsteps;
   return CULBM_OK;
 }
 
@@ -563,6 +600,7 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
   const culbm_config& c = ctx->cfg;
   ctx->n = int64_t(c.nx) * c.ny * c.nz;
   ctx->inplace = c.stream_policy == CULBM_STREAM_INPLACE;
+  ctx->sr = c.rounding == CULBM_ROUND_STOCHASTIC;
   ctx->bpp = c.storage == CULBM_STORAGE_FP32 ? 4 : 2;
   ctx->scale = c.storage == CULBM_STORAGE_FP32 ? 1.0f
                : (c.storage_scale != 0.0f ? c.storage_scale : default_scale(c.storage));
@@ -601,7 +639,7 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
     int per_cu = 1;
     by_storage(c.storage, [&](auto s) {
       using S = decltype(s);
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused2<S>),
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused2<S, false>),
                                                        kFBlock, 0) != hipSuccess)
         per_cu = 1;
       return 0;

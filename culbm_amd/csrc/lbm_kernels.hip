@@ -115,18 +115,41 @@ __device__ __forceinline__ PullGather<S> make_gather(const StepArgs& a, const Ce
   return g;
 }
 
-template <class S, int I>
-__device__ __forceinline__ void store_plain(char* base, int64_t dsb, uint32_t lb, const float* f,
-                                            float s) {
-  st_at<typename S::T>(base + I * dsb, lb, S::enc(f[I], s));
-  if constexpr (I < 26) store_plain<S, I + 1>(base, dsb, lb, f, s);
+// Storage encoding of direction I of one cell: round to nearest, or (SR)
+// stochastically with the cell's random bytes.
+template <class S, bool SR>
+struct Encoder {
+  float s;
+  SrHash h;  // used only when SR
+  template <int I>
+  __device__ __forceinline__ typename S::T get(float v) const {
+    if constexpr (SR)
+      return S::enc_sr(v, s, h.template byte<I>());
+    else
+      return S::enc(v, s);
+  }
+};
+
+template <class S, bool SR>
+__device__ __forceinline__ Encoder<S, SR> make_encoder(const StepArgs& a, uint32_t cell, uint32_t state) {
+  Encoder<S, SR> e;
+  e.s = a.scale;
+  if constexpr (SR) e.h.make(cell, state);
+  return e;
 }
 
-template <class S>
+template <class S, bool SR, int I>
+__device__ __forceinline__ void store_plain(char* base, int64_t dsb, uint32_t lb, const float* f,
+                                            const Encoder<S, SR>& e) {
+  st_at<typename S::T>(base + I * dsb, lb, e.template get<I>(f[I]));
+  if constexpr (I < 26) store_plain<S, SR, I + 1>(base, dsb, lb, f, e);
+}
+
+template <class S, bool SR = false>
 __device__ __forceinline__ void store_plain_all(const StepArgs& a, const Cell& k, void* dst,
-                                                const float* f) {
+                                                const float* f, const Encoder<S, SR>& e) {
   constexpr uint32_t B = sizeof(typename S::T);
-  store_plain<S, 0>(static_cast<char*>(dst), a.dstride * B, k.l * B, f, a.scale);
+  store_plain<S, SR, 0>(static_cast<char*>(dst), a.dstride * B, k.l * B, f, e);
 }
 
 // Cells without LOAD start from zero populations.
@@ -143,7 +166,7 @@ __device__ __forceinline__ void update_cell(uint32_t fl, const StepArgs& a, uint
   if (fl & kCollide) collide(a.inv_tau, a.omt, f);
 }
 
-template <class S>
+template <class S, bool SR>
 __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
   if (!k.ok) return;
@@ -154,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   const uint32_t fl = a.sflag[k.c];
   if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, f);
-  if (fl & kStore) store_plain_all<S>(a, k, a.dst, f);
+  if (fl & kStore) store_plain_all<S, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
 
 // ---------------------------------------------------------------------------
@@ -198,18 +221,18 @@ struct TwistLoad {
 };
 
 // Stores of a step of parity P (1 = O: slot 26-i, 0 = E: slot i) at x + p(c_i).
-template <class S, int P>
+template <class S, int P, bool SR>
 struct TwistStore {
   char* base;
   int64_t dsb;
   TwistRows r;
-  float s;
+  Encoder<S, SR> e;
   template <int I>
   __device__ __forceinline__ void run(const float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? 26 - I : I;
     constexpr uint32_t ox = cx == 1 ? sizeof(typename S::T) : 0;
-    st_at<typename S::T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, S::enc(f[I], s));
+    st_at<typename S::T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, e.template get<I>(f[I]));
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
@@ -217,12 +240,12 @@ struct TwistStore {
 // Ghost copies of a face cell: a consumer is clamped onto this cell on axis a
 // when x_a - c_a leaves the domain (c_a = +1 on the low face, -1 on the high
 // face); its load slot on that axis is at p(-c_a) instead of p(c_a).
-template <class S, int P>
+template <class S, int P, bool SR>
 struct TwistGhost {
   char* base;
   int64_t dsb;
   TwistRows r;
-  float s;
+  Encoder<S, SR> e;
   uint32_t fx, fy, fz;  // bit 0: coordinate 0 on the axis; bit 1: coordinate n-1
   template <int I>
   __device__ __forceinline__ void run(const float* f) const {
@@ -234,7 +257,7 @@ struct TwistGhost {
     const uint32_t gz = cz == 1 ? (fz & 1u) : (cz == -1 ? (fz >> 1) : 0u);
     const uint32_t gm = gx | (gy << 1) | (gz << 2);
     if (gm) {
-      const typename S::T v = S::enc(f[I], s);
+      const typename S::T v = e.template get<I>(f[I]);  // the value of the regular store
       char* p = base + slot * dsb;
       const int32_t sx = cx == 1 ? B : 0, sy = cy == 1 ? r.row : 0, sz = cz == 1 ? r.plane : 0;
       const int32_t qx = cx == -1 ? B : 0, qy = cy == -1 ? r.row : 0, qz = cz == -1 ? r.plane : 0;
@@ -258,24 +281,24 @@ __device__ __forceinline__ void twist_load_all(const StepArgs& a, const Cell& k,
   t.template run<0>(f);
 }
 
-template <class S, int P>
+template <class S, int P, bool SR = false>
 __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k, void* buf,
-                                                const float* f) {
-  TwistStore<S, P> t;
+                                                const float* f, const Encoder<S, SR>& e) {
+  TwistStore<S, P, SR> t;
   t.base = static_cast<char*>(buf);
   t.dsb = a.dstride * int64_t(sizeof(typename S::T));
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
-  t.s = a.scale;
+  t.e = e;
   t.template run<0>(f);
   const uint32_t fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
   const uint32_t fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
   const uint32_t fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
   if (fx | fy | fz) {  // one branch per wave; interior waves skip it entirely
-    TwistGhost<S, P> g;
+    TwistGhost<S, P, SR> g;
     g.base = t.base;
     g.dsb = t.dsb;
     g.r = t.r;
-    g.s = a.scale;
+    g.e = e;
     g.fx = fx;
     g.fy = fy;
     g.fz = fz;
@@ -283,7 +306,7 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   }
 }
 
-template <class S, int P>
+template <class S, int P, bool SR>
 __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
   if (!k.ok) return;
@@ -294,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   const uint32_t fl = a.sflag[k.c];
   if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, f);
-  if (fl & kStore) twist_store_all<S, P>(a, k, a.dst, f);
+  if (fl & kStore) twist_store_all<S, P, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
 
 // ---------------------------------------------------------------------------
@@ -360,9 +383,15 @@ __device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, in
   q.fl = a.sflag[k.c];
 }
 
-// First step of a gathered A cell into the ring: a plain pull update, or the
-// cell's own populations when it has no STORE.
-template <class S>
+template <class S, bool SR, int I>
+__device__ __forceinline__ void encode_all(const float* f, typename S::T* v, const Encoder<S, SR>& e) {
+  v[I] = e.template get<I>(f[I]);
+  if constexpr (I < 26) encode_all<S, SR, I + 1>(f, v, e);
+}
+
+// First step of a gathered A cell into the ring: a plain pull update (the state
+// numbered a.state), or the cell's own populations when it has no STORE.
+template <class S, bool SR>
 __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_t p, APending& q,
                                                FusedRing<typename S::T>& r) {
   using T = typename S::T;
@@ -371,8 +400,7 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   update_cell(q.fl, a, q.c, q.f);
   T v[27];
   if (q.fl & kStore) {
-#pragma unroll
-    for (int i = 0; i < 27; ++i) v[i] = S::enc(q.f[i], a.scale);
+    encode_all<S, SR, 0>(q.f, v, make_encoder<S, SR>(a, q.c, a.state));
   } else {
     constexpr uint32_t B = sizeof(T);
     const char* src = static_cast<const char*>(a.src);
@@ -388,7 +416,7 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   }
 }
 
-template <class S>
+template <class S, bool SR>
 __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
   using T = typename S::T;
   __shared__ FusedRing<T> r;
@@ -409,7 +437,7 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
   for (int32_t p = max(zs - 1, 0); p <= zs; ++p) {
     if (a_ok) {
       fused_a_gather<S>(fa.a, x0, y0, t, p, q);
-      fused_a_finish<S>(fa.a, t, p, q, r);
+      fused_a_finish<S, SR>(fa.a, t, p, q, r);
     }
     __syncthreads();  // the dn group of plane zs-1 is overwritten by plane zs
   }
@@ -423,7 +451,7 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
   const int lym = gy == 0 ? by + 1 : by, lyp = gy == ny - 1 ? by + 1 : by + 2;
   for (int32_t z = zs; z < ze; ++z) {
     if (a_ok && z + 1 < nz) {
-      fused_a_finish<S>(fa.a, t, z + 1, q, r);
+      fused_a_finish<S, SR>(fa.a, t, z + 1, q, r);
       if (z + 1 < ze && z + 2 < nz) fused_a_gather<S>(fa.a, x0, y0, t, z + 2, q);
     }
     __syncthreads();
@@ -450,7 +478,8 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
       const uint32_t fl = fa.a.sflag[k.c];
       if (!(fl & kLoad)) clear(f);
       update_cell(fl, fa.a, k.c, f);
-      if (fl & kStore) store_plain_all<S>(fa.a, k, fa.a.dst, f);
+      if (fl & kStore)  // the second step produces state a.state + 1
+        store_plain_all<S, SR>(fa.a, k, fa.a.dst, f, make_encoder<S, SR>(fa.a, k.c, fa.a.state + 1));
     }
     __syncthreads();
   }
@@ -500,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_to_pull(const StepArgs a, cons
   float f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) f[i] = plain[i * n + k.c];
-  store_plain_all<S>(a, k, out, f);
+  store_plain_all<S>(a, k, out, f, make_encoder<S, false>(a, 0u, 0u));
 }
 
 template <class S>
@@ -524,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_to_twist(const StepArgs a, con
   float f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) f[i] = plain[i * n + k.c];
-  twist_store_all<S, P>(a, k, out, f);
+  twist_store_all<S, P>(a, k, out, f, make_encoder<S, false>(a, 0u, 0u));
 }
 
 // in-place layout left by a step of parity P -> plain FP32 (regular slots)

@@ -36,11 +36,40 @@ struct FastDiv {
 // exact apart from the single rounding.
 enum Storage : int32_t { kFP32 = 0, kFP16 = 1, kBF16 = 2 };
 
+// Stochastic rounding of the 16-bit formats (opt-in).  Every value a step
+// stores gets one random byte b from a hash of (cell, direction, number of the
+// state the step produces), the same in every kernel and layout, so all
+// variants stay bit-identical; b decides the rounding direction with
+// probability equal to the value's position between its two neighbours in the
+// format (to 1/256).
+__host__ __device__ inline uint32_t mix32(uint32_t x) {  // two multiply-xorshift rounds
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+struct SrHash {
+  uint32_t h[7];  // 4 random bytes each, one per direction
+  __device__ __forceinline__ void make(uint32_t cell, uint32_t state) {
+    const uint32_t key = mix32((cell * 0x9e3779b1u) ^ mix32(state * 0x85ebca77u + 0x632be5abu));
+#pragma unroll
+    for (int j = 0; j < 7; ++j) h[j] = mix32(key + uint32_t(j) * 0x27d4eb2fu);
+  }
+  template <int I>
+  __device__ __forceinline__ uint32_t byte() const {
+    return (h[I / 4] >> (8 * (I % 4))) & 0xffu;
+  }
+};
+
 struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
   __device__ __forceinline__ static float dec(T v, float) { return v; }
   __device__ __forceinline__ static T enc(float v, float) { return v; }
+  __device__ __forceinline__ static T enc_sr(float v, float, uint32_t) { return v; }
 };
 
 struct StoreF16 {
@@ -51,6 +80,13 @@ struct StoreF16 {
   // and the conversion into one mixed-precision FMA with a +0 addend, which
   // does the same; writing it out makes every code path agree.
   __device__ __forceinline__ static T enc(float v, float s) { return T(v * s + 0.0f); }
+  // stochastic: add b * 2^5 + 2^4 below the 10 kept mantissa bits of the FP32
+  // value, clear the 13 dropped bits, convert (exact in the binary16 normal
+  // range; below it the conversion rounds to the subnormal grid)
+  __device__ __forceinline__ static T enc_sr(float v, float s, uint32_t b) {
+    const uint32_t u = (__float_as_uint(v * s) + (b << 5) + 0x10u) & ~0x1fffu;
+    return T(__uint_as_float(u));
+  }
 };
 
 struct StoreBF16 {
@@ -62,6 +98,11 @@ struct StoreBF16 {
   __device__ __forceinline__ static T enc(float v, float s) {
     uint32_t u = __float_as_uint(v * s);
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return T(u >> 16);
+  }
+  // stochastic: add b * 2^8 + 2^7 to the FP32 bits, then truncate
+  __device__ __forceinline__ static T enc_sr(float v, float s, uint32_t b) {
+    const uint32_t u = __float_as_uint(v * s) + (b << 8) + 0x80u;
     return T(u >> 16);
   }
 };
@@ -89,6 +130,7 @@ struct StepArgs {
   void* dst;  // pull: destination buffer; in-place: same as src
   float inv_tau, omt;
   float scale, inv_scale;
+  uint32_t state;        // number of the state this launch produces (stochastic rounding key)
 };
 
 }  // namespace culbm

@@ -7,7 +7,7 @@
 // command line goes through the C ABI of include/culbm.h.
 //
 // Options added by this engine: --storage {fp32,fp16,bf16}, --storageScale S,
-// --waveDepth D, --initDDF {rest,feq}, --json.
+// --waveDepth D, --initDDF {rest,feq}, --rounding {nearest,stochastic}, --json.
 #include <culbm.h>
 
 #include <cerrno>
@@ -65,6 +65,7 @@ const char* kUsage =
     "  --storageScale FLOAT      16-bit storage scale (power of two, 0 = default) [0]\n"
     "  --waveDepth INT           Planes per temporal-blocking chunk (0 = automatic) [0]\n"
     "  --initDDF TEXT            Populations at start {rest,feq} [rest]\n"
+    "  --rounding TEXT           16-bit stores {nearest,stochastic} [nearest]\n"
     "  --json                    Also print one JSON line per window\n"
     "  --dryRun                  Validate, print the parsed configuration as JSON and exit\n";
 
@@ -209,6 +210,11 @@ Options parse(int argc, char** argv) {
       if (v == "rest") o.init_ddf = CULBM_INIT_REST;
       else if (v == "feq") o.init_ddf = CULBM_INIT_FEQ;
       else throw ParseError("--initDDF: expected rest or feq, got " + v);
+    } else if (opt == "--rounding") {
+      const std::string v = value();
+      if (v == "nearest") o.cfg.rounding = CULBM_ROUND_NEAREST;
+      else if (v == "stochastic") o.cfg.rounding = CULBM_ROUND_STOCHASTIC;
+      else throw ParseError("--rounding: expected nearest or stochastic, got " + v);
     } else if (opt == "--json") {
       o.json = flag();
     } else if (opt == "--dryRun") {
@@ -285,10 +291,10 @@ class Simulator {
         "{\"domainDim\": [%d, %d, %d], \"blockDim\": [%d, %d, %d], \"gridDim\": [%d, %d, %d], "
         "\"invTau\": %.9g, \"streamPolicy\": %d, \"optPolicy\": %d, \"innerLoop\": %d, "
         "\"nstep\": %d, \"dstep\": %d, \"devId\": %d, \"useRingDDFBuf\": %s, \"storage\": %d, "
-        "\"initStateFolder\": \"%s\", \"dumpFolder\": \"%s\", \"dump\": [%d, %d, %d, %d]}\n",
+        "\"rounding\": %d, \"initStateFolder\": \"%s\", \"dumpFolder\": \"%s\", \"dump\": [%d, %d, %d, %d]}\n",
         c.nx, c.ny, c.nz, c.block_dim[0], c.block_dim[1], c.block_dim[2], c.grid_dim[0], c.grid_dim[1],
         c.grid_dim[2], double(c.inv_tau), c.stream_policy, c.opt_policy, c.inner_loop, o_.nstep,
-        o_.dstep, c.device, c.use_ring ? "true" : "false", c.storage, o_.init_folder.c_str(),
+        o_.dstep, c.device, c.use_ring ? "true" : "false", c.storage, c.rounding, o_.init_folder.c_str(),
         o_.dump_folder.c_str(), int(o_.dump_rho), int(o_.dump_vx), int(o_.dump_vy), int(o_.dump_vz));
   }
   ~Simulator() { culbm_destroy(ctx_); }

@@ -17,6 +17,7 @@ import numpy as np
 from . import _lib
 
 STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
+ROUNDING = {"nearest": 0, "stochastic": 1}
 INIT = {"rest": 0, "feq": 1}
 
 
@@ -36,6 +37,7 @@ class LatticeConfig:
     storage: str = "fp32"
     storage_scale: float = 0.0
     wave_depth: int = 0
+    rounding: str = "nearest"                       # 16-bit stores: "nearest" or "stochastic"
 
     def to_c(self) -> _lib.Config:
         c = _lib.Config()
@@ -54,6 +56,9 @@ class LatticeConfig:
         c.storage = STORAGE[self.storage]
         c.storage_scale = float(self.storage_scale)
         c.wave_depth = int(self.wave_depth)
+        if self.rounding not in ROUNDING:
+            raise ValueError(f"rounding must be one of {sorted(ROUNDING)}, got {self.rounding!r}")
+        c.rounding = ROUNDING[self.rounding]
         return c
 
 

@@ -68,6 +68,9 @@ extern "C" {
 #define CULBM_STORAGE_FP32 0
 #define CULBM_STORAGE_FP16 1
 #define CULBM_STORAGE_BF16 2
+/* rounding of the 16-bit stores (no reference counterpart) */
+#define CULBM_ROUND_NEAREST 0    /* round to nearest even                       */
+#define CULBM_ROUND_STOCHASTIC 1 /* unbiased, keyed by (cell, direction, step)  */
 /* culbm_init_ddf modes */
 #define CULBM_INIT_REST 0 /* feq(1,0) = 0 everywhere, as the reference   */
 #define CULBM_INIT_FEQ 1  /* feq(rho, u) of the rho/u fields (opt-in)     */
@@ -89,7 +92,8 @@ typedef struct culbm_config {
   int32_t storage;          /* CULBM_STORAGE_*                               */
   float storage_scale;      /* 16-bit scale (power of two); 0 = default      */
   int32_t wave_depth;       /* planes per wavefront chunk; 0 = automatic     */
-  int32_t reserved[7];
+  int32_t rounding;         /* CULBM_ROUND_* of the 16-bit stores            */
+  int32_t reserved[6];
 } culbm_config;
 
 typedef struct culbm_info {

@@ -217,6 +217,35 @@ inline uint16_t f32_to_bf16(float f) {
   return uint16_t(u >> 16);
 }
 
+// ---- stochastic rounding of the 16-bit formats -------------------------------
+// Each value a step stores gets one random byte b from a hash of (cell, direction,
+// number of the state produced); the value is rounded up with probability equal
+// to its position between the two neighbouring 16-bit values (to 1/256):
+//  * binary16: add b * 2^5 + 2^4 to the FP32 bits, clear the 13 bits binary16
+//    drops, convert (exact in the binary16 normal range);
+//  * bfloat16: add b * 2^8 + 2^7 to the FP32 bits, then truncate.
+inline uint32_t mix32(uint32_t x) {  // a 32-bit integer finaliser (two multiply-xorshift rounds)
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+inline uint32_t sr_key(uint32_t cell, uint32_t state) {
+  return mix32((cell * 0x9e3779b1u) ^ mix32(state * 0x85ebca77u + 0x632be5abu));
+}
+inline uint32_t sr_byte(uint32_t key, int i) {
+  return (mix32(key + uint32_t(i / 4) * 0x27d4eb2fu) >> (8 * (i % 4))) & 0xffu;
+}
+inline uint16_t f32_to_f16_sr(float t, uint32_t b) {
+  return f32_to_f16(bitsf((fbits(t) + (b << 5) + 0x10u) & ~0x1fffu));
+}
+inline uint16_t f32_to_bf16_sr(float t, uint32_t b) {
+  const uint32_t u = fbits(t) + (b << 8) + 0x80u;
+  return uint16_t(u >> 16);
+}
+
 inline float bf16_to_f32(uint16_t h) { return bitsf(uint32_t(h) << 16); }
 
 // ---- lattice helpers -------------------------------------------------------
@@ -396,6 +425,36 @@ void orc_step_s16(int nx, int ny, int nz, const uint32_t* flag, const float* rho
         dst[int64_t(i) * d.n + c] = fmt == 1 ? f32_to_f16(t + 0.0f) : f32_to_bf16(t);
       },
       nthreads);
+}
+
+// orc_step_s16 with stochastic rounding of the stored values; `state` is the
+// number of the state the step produces (the step counter after the step).
+void orc_step_s16_sr(int nx, int ny, int nz, const uint32_t* flag, const float* rho, const float* vx,
+                     const float* vy, const float* vz, float inv_tau, int fmt, float scale,
+                     uint32_t state, const uint16_t* src, uint16_t* dst, int nthreads) {
+  const Dom d{nx, ny, nz, int64_t(nx) * ny * nz};
+  const float inv_scale = 1.0f / scale;
+  step_generic<float>(
+      d, flag, rho, vx, vy, vz, inv_tau,
+      [&](int i, int64_t s) {
+        const uint16_t h = src[int64_t(i) * d.n + s];
+        return (fmt == 1 ? f16_to_f32(h) : bf16_to_f32(h)) * inv_scale;
+      },
+      [&](int i, int64_t c, float v) {
+        const float t = v * scale;
+        const uint32_t b = sr_byte(sr_key(uint32_t(c), state), i);
+        dst[int64_t(i) * d.n + c] = fmt == 1 ? f32_to_f16_sr(t, b) : f32_to_bf16_sr(t, b);
+      },
+      nthreads);
+}
+
+// Vector form for the codec tests: encode already-scaled values with the
+// random bytes of (cell k mod cells, direction k / cells) for `state`.
+void orc_encode16_sr(int fmt, const float* in, uint16_t* out, int64_t n, int64_t cells, uint32_t state) {
+  for (int64_t k = 0; k < n; ++k) {
+    const uint32_t b = sr_byte(sr_key(uint32_t(k % cells), state), int(k / cells));
+    out[k] = fmt == 1 ? f32_to_f16_sr(in[k], b) : f32_to_bf16_sr(in[k], b);
+  }
 }
 
 // Dump: rho/u from the pull-gathered populations of src, written only where

@@ -44,6 +44,9 @@ _SIG = {
     "orc_step_f32": [_i, _i, _i, _U32, _F32, _F32, _F32, _F32, ctypes.c_float, _F32, _F32, _i],
     "orc_step_s16": [_i, _i, _i, _U32, _F32, _F32, _F32, _F32, ctypes.c_float, _i, ctypes.c_float,
                      _U16, _U16, _i],
+    "orc_step_s16_sr": [_i, _i, _i, _U32, _F32, _F32, _F32, _F32, ctypes.c_float, _i, ctypes.c_float,
+                        ctypes.c_uint32, _U16, _U16, _i],
+    "orc_encode16_sr": [_i, _F32, _U16, _i64, _i64, ctypes.c_uint32],
     "orc_dump_f32": [_i, _i, _i, _U32, _F32, _F32, _F32, _F32, _F32, _i],
     "orc_dump_s16": [_i, _i, _i, _U32, _i, ctypes.c_float, _U16, _F32, _F32, _F32, _F32, _i],
     "orc_dump_f64": [_i, _i, _i, _U32, _F64, _F64, _F64, _F64, _F64, _i],
@@ -155,7 +158,7 @@ class Sim:
     dtype float64: accuracy oracle; float32: bit-exact FP32 restatement;
     'fp16'/'bf16': FP32 arithmetic over 16-bit shifted storage."""
 
-    def __init__(self, state, inv_tau, dtype=np.float64, scale=1.0, nthreads=None):
+    def __init__(self, state, inv_tau, dtype=np.float64, scale=1.0, nthreads=None, rounding="nearest"):
         self.flag = np.ascontiguousarray(state.flag, np.uint32)
         self.nz, self.ny, self.nx = self.flag.shape
         self.fields = _fields(state)
@@ -168,6 +171,7 @@ class Sim:
         self.src = np.zeros(shp, st)
         self.dst = np.zeros(shp, st)
         self.step_count = 0
+        self.stochastic = rounding == "stochastic"
 
     # populations as float arrays (decoded)
     def ddf(self) -> np.ndarray:
@@ -208,6 +212,10 @@ class Sim:
             elif self.kind == "float32":
                 lib().orc_step_f32(self.nx, self.ny, self.nz, self.flag, rho, vx, vy, vz,
                                    float(self.inv_tau), self.src, self.dst, self.nthreads)
+            elif self.stochastic:
+                lib().orc_step_s16_sr(self.nx, self.ny, self.nz, self.flag, rho, vx, vy, vz,
+                                      float(self.inv_tau), 1 if self.kind == "fp16" else 2, self.scale,
+                                      self.step_count + 1, self.src, self.dst, self.nthreads)
             else:
                 lib().orc_step_s16(self.nx, self.ny, self.nz, self.flag, rho, vx, vy, vz,
                                    float(self.inv_tau), 1 if self.kind == "fp16" else 2, self.scale,

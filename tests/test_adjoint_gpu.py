@@ -38,9 +38,9 @@ def make_case(seed: int = 5, frozen: bool = True):
     return st, w
 
 
-def lattice(st, storage="fp32", sp=0, op=0, inner=5):
+def lattice(st, storage="fp32", sp=0, op=0, inner=5, rounding="nearest"):
     lat = Lattice(LatticeConfig(domain=st.domain, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
-                                inner_loop=inner, storage=storage))
+                                inner_loop=inner, storage=storage, rounding=rounding))
     lat.set_state(st.flag, st.rho, st.vx, st.vy, st.vz)
     lat.init_ddf("feq")
     return lat
@@ -113,13 +113,20 @@ def test_schedules_give_identical_bits(gpu_fp32):
             assert np.array_equal(getattr(r, key), getattr(gpu_fp32, key)), (kw, key)
 
 
-@pytest.mark.parametrize("sp,op", [(0, 0), (1, 0), (1, 3), (0, 2)])
-def test_context_ends_nsteps_on(sp, op):
+@pytest.mark.parametrize("sp,op,storage,rounding", [(0, 0, "fp32", "nearest"), (1, 0, "fp32", "nearest"),
+                                                   (1, 3, "fp32", "nearest"), (0, 2, "fp32", "nearest"),
+                                                   (1, 3, "fp16", "stochastic"), (0, 2, "bf16", "stochastic")])
+def test_context_ends_nsteps_on(sp, op, storage, rounding):
+    """The adjoint's forward run reproduces the context's own trajectory bit for
+    bit (with stochastic rounding it continues the context's step numbers)."""
     st, w = make_case(frozen=False)
-    with lattice(st, sp=sp, op=op, inner=4) as a, lattice(st, sp=sp, op=op, inner=4) as b:
+    kw = dict(sp=sp, op=op, inner=4, storage=storage, rounding=rounding)
+    with lattice(st, **kw) as a, lattice(st, **kw) as b:
+        a.step(3)
+        b.step(3)
         a.adjoint(w, N, (T0, T1), checkpoint_every=5)
         b.step(N)
-        assert a.info()["step"] == b.info()["step"] == N
+        assert a.info()["step"] == b.info()["step"] == N + 3
         assert np.array_equal(a.get_ddf(), b.get_ddf())
         a.step(3)
         b.step(3)

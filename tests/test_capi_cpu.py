@@ -61,6 +61,7 @@ def test_defaults_are_the_reference_cli_defaults():
     (dict(domain=(300, 64, 64), opt_policy=3, block_dim=(2, 1, 1), grid_dim=(1, 1, 1), inner_loop=4),
      "<Domain Dimension, Blocking Dimension and InnerLoop> don't match!"),
     (dict(inv_tau=float("nan")), "finite"),
+    (dict(storage="fp32", rounding="stochastic"), "stochastic rounding applies to FP16 or BF16"),
 ])
 def test_validation_errors(kw, msg):
     err = validate(LatticeConfig(**kw))
@@ -71,6 +72,8 @@ def test_valid_configs():
     for sp, op in [(0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (0, 3), (1, 3)]:
         assert validate(LatticeConfig(domain=(64, 64, 64), stream_policy=sp, opt_policy=op)) is None
     assert validate(LatticeConfig(domain=(64, 64, 64), stream_policy=1, opt_policy=3, use_ring=True)) is None
+    for storage in ("fp16", "bf16"):
+        assert validate(LatticeConfig(domain=(64, 64, 64), storage=storage, rounding="stochastic")) is None
 
 
 def test_create_reports_validation_error_without_device():

@@ -102,21 +102,22 @@ def test_plain_pull_small_config_vs_oracle():
     assert np.abs(vx64[fluid]).max() > 1e-3
 
 
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
 @pytest.mark.parametrize("storage,fmt", [("fp16", "fp16"), ("bf16", "bf16")])
-def test_plain_pull_16bit_vs_oracle(storage, fmt):
+def test_plain_pull_16bit_vs_oracle(storage, fmt, rounding):
     st = cases.config_small(seed=7, n=40)
-    lat = make(st, init="feq", inv_tau=0.6, storage=storage)
+    lat = make(st, init="feq", inv_tau=0.6, storage=storage, rounding=rounding)
     scale = lat.info()["storage_scale"]
     lat.step(30)
     lat.dump()
     g = lat.macros()
-    o = pyoracle.Sim(st, 0.6, fmt, scale=scale)
+    o = pyoracle.Sim(st, 0.6, fmt, scale=scale, rounding=rounding)
     o.init_feq()
     o.step(30)
     r = o.dump()
-    assert_bitexact(lat.get_ddf(), o.ddf(), f"{storage} populations")
+    assert_bitexact(lat.get_ddf(), o.ddf(), f"{storage} {rounding} populations")
     for a, b in zip(g, r):
-        assert_bitexact(a, b, f"{storage} macros")
+        assert_bitexact(a, b, f"{storage} {rounding} macros")
 
 
 # ---- streaming KAT: index tags through pure streaming ------------------------
@@ -171,12 +172,13 @@ def obstacle_case(nx=37, ny=29, nz=31, outlet="zero_gradient", seed=11):
     return st
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("storage,rounding", [("fp32", "nearest"), ("fp16", "nearest"), ("bf16", "nearest"),
+                                              ("fp16", "stochastic"), ("bf16", "stochastic")])
 @pytest.mark.parametrize("outlet", ["zero_gradient", "equilibrium"])
-def test_variants_bitexact_vs_plain(storage, outlet):
+def test_variants_bitexact_vs_plain(storage, rounding, outlet):
     st = obstacle_case(outlet=outlet)
     steps = 17
-    ref = make(st, init="feq", inv_tau=0.7, storage=storage)
+    ref = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding)
     ref.step(steps)
     ref.dump()
     want_m = ref.macros()
@@ -184,11 +186,11 @@ def test_variants_bitexact_vs_plain(storage, outlet):
     for sp, op, il, wd in [(1, 0, 5, 0), (0, 3, 5, 3), (1, 3, 5, 3), (1, 3, 1, 2), (1, 3, 4, 0),
                            (0, 1, 3, 4), (1, 1, 5, 2), (1, 1, 2, 0), (0, 2, 5, 0), (0, 2, 2, 3),
                            (0, 2, 4, 1), (0, 2, 3, 7)]:
-        lat = make(st, init="feq", inv_tau=0.7, storage=storage, stream_policy=sp, opt_policy=op,
-                   inner_loop=il, wave_depth=wd)
+        lat = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding, stream_policy=sp,
+                   opt_policy=op, inner_loop=il, wave_depth=wd)
         lat.step(steps)
         lat.dump()
-        what = f"({sp},{op}) I={il} D={wd} {storage} {outlet}"
+        what = f"({sp},{op}) I={il} D={wd} {storage} {rounding} {outlet}"
         assert_bitexact(lat.get_ddf(), want_f, what + " populations")
         for a, b in zip(lat.macros(), want_m):
             assert_bitexact(a, b, what + " macros")

@@ -238,3 +238,51 @@ def test_bf16_codec_round_to_nearest_even():
     dec = np.zeros(x.size, np.float32)
     po.lib().orc_decode16(2, enc, dec, x.size)
     assert np.all(np.abs(dec - x) <= np.abs(x) * 2.0 ** -8)
+
+
+# ---- stochastic rounding of the 16-bit formats ---------------------------------
+
+def _sr_encode(fmt, x, cells, state):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.size, np.uint16)
+    po.lib().orc_encode16_sr(1 if fmt == "fp16" else 2, x.reshape(-1), out, x.size, cells, state)
+    return out
+
+
+def _decode(fmt, h):
+    out = np.zeros(h.size, np.float32)
+    po.lib().orc_decode16(1 if fmt == "fp16" else 2, h, out, h.size)
+    return out
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+def test_stochastic_rounding_picks_a_neighbour_and_is_unbiased(fmt):
+    rng = np.random.default_rng(5)
+    cells = 4096
+    x = rng.uniform(-40.0, 40.0, cells).astype(np.float32)
+    near = np.zeros(cells, np.uint16)
+    po.lib().orc_encode16(1 if fmt == "fp16" else 2, x, near, cells)
+    acc = np.zeros(cells)
+    states = 400
+    # the format's quantum at x: 2^(floor(log2|x|) - mantissa bits)
+    quantum = 2.0 ** (np.floor(np.log2(np.abs(x.astype(np.float64)))) - (10 if fmt == "fp16" else 7))
+    for state in range(1, states + 1):
+        v = _decode(fmt, _sr_encode(fmt, x, cells, state)).astype(np.float64)
+        # always one of the two representable neighbours of x
+        assert np.all(np.abs(v - x) < quantum)
+        acc += v
+    # the mean over independent draws approaches x (unbiased to the 1/256 resolution),
+    # which round to nearest, always on the same side, does not
+    err = np.abs(acc / states - x)
+    rn_err = np.abs(_decode(fmt, near).astype(np.float64) - x)
+    assert np.median(err / quantum) < 0.05
+    assert err.mean() < 0.25 * rn_err.mean()
+
+
+def test_stochastic_rounding_is_a_function_of_cell_direction_and_state():
+    x = np.linspace(-3.0, 3.0, 27 * 64).astype(np.float32)
+    a = _sr_encode("fp16", x, 64, 7)
+    b = _sr_encode("fp16", x, 64, 7)
+    c = _sr_encode("fp16", x, 64, 8)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
