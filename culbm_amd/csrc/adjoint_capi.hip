@@ -6,7 +6,8 @@
 // "slots" of the dense pull layout.  The reverse sweep needs f^t for t = N-1 .. 0
 // and recomputes what no slot holds:
 //  * every-k: the sweep keeps f^0, f^k, f^2k, ...; each segment is recomputed
-//    once from its checkpoint into k-1 segment slots and reversed;
+//    once from its checkpoint into k-1 segment slots and reversed, except the
+//    last, whose states the sweep itself leaves in the segment slots;
 //  * binomial: with s slots, reverse(a, b) stores a state at the split m that
 //    minimises the forward steps of reverse(m, b) with one slot fewer plus
 //    reverse(a, m) (Griewank's binomial checkpointing, "revolve"); the
@@ -109,7 +110,7 @@ AdjPlan adj_make_plan(int32_t nsteps, int32_t every, int32_t slots) {
     const int32_t C = (nsteps + every - 1) / every;
     const int32_t seg = std::min(every, nsteps) - 1;
     p.slots = C + seg;
-    AdjPlanner pl{p, {}, false};
+    AdjPlanner pl{p, {}, true};
     for (int32_t s = C + seg - 1; s >= C; --s) pl.free.push_back(s);
     for (int32_t j = 1; j < C; ++j) p.sweep.push_back({j * every, j});
     for (int32_t j = C - 1; j >= 0; --j) pl.rev(j * every, std::min(nsteps, (j + 1) * every), j);

@@ -355,28 +355,32 @@ struct FusedRing {
   T up[3][9][kFA];  // directions 18..26 (c_z = +1), slot = plane % 3
   T mid[2][9][kFA]; // directions 9..17 (c_z = 0), slot = plane % 2
   T dn[9][kFA];     // directions 0..8 (c_z = -1), the latest plane
+  uint8_t fl[2][kFA];  // step bits of the A cells, slot = plane % 2 (the B step's flags)
 };
 
 // Pulled (not yet updated) populations of one A cell, gathered a plane ahead
 // so the HBM latency overlaps the previous plane's second step.
 struct APending {
   float f[27];
-  uint32_t fl;
+  uint8_t fl;  // kept 8-bit: a widened copy across the loop would wait for every load
   uint32_t c;
   bool ok;
 };
 
 template <class S>
+// Every lane gathers, at a position clamped into the domain (the caller clamps
+// the plane), and only q.ok tells whether the cell is used: loads issued on
+// every path let the compiler wait for exactly one gather while the next one
+// is still in flight, instead of for every outstanding load.
 __device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, int32_t y0, int ac,
                                                int32_t p, APending& q) {
   const int32_t gx = x0 - 1 + ac % kFAX, gy = y0 - 1 + ac / kFAX;
-  q.ok = gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny;
-  if (!q.ok) return;
+  q.ok = ac < kFA && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny;
   Cell k;
-  k.x = gx;
-  k.y = gy;
+  k.x = min(max(gx, 0), a.nx - 1);
+  k.y = min(max(gy, 0), a.ny - 1);
   k.z = p;
-  k.c = (uint32_t(p) * uint32_t(a.ny) + uint32_t(gy)) * uint32_t(a.nx) + uint32_t(gx);
+  k.c = (uint32_t(p) * uint32_t(a.ny) + uint32_t(k.y)) * uint32_t(a.nx) + uint32_t(k.x);
   k.l = k.c;
   q.c = k.c;
   make_gather<S>(a, k, a.src).template run<0>(q.f);
@@ -408,6 +412,7 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
     for (int i = 0; i < 27; ++i) v[i] = ld_at<T>(src + i * (a.dstride * B), q.c * B);
   }
   const int su = p % 3, sm = p % 2;
+  r.fl[sm][ac] = uint8_t(q.fl);
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     r.dn[j][ac] = v[j];
@@ -417,7 +422,9 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
 }
 
 template <class S, bool SR>
-__global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
+// Three waves per SIMD: two workgroups per CU, as many as the FP32 ring's LDS allows.
+__global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fused2(
+    const FusedArgs fa) {
   using T = typename S::T;
   __shared__ FusedRing<T> r;
   const uint32_t ntile = uint32_t(fa.tiles_x) * fa.tiles_y * fa.nchunk;
@@ -431,17 +438,21 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
   const int t = threadIdx.x;
   const bool a_ok = t < kFA;
 
-  APending q;
-  q.ok = false;
-  // prologue: A planes zs-1 (if any) and zs, then the gather of plane zs+1
+  // A planes gathered ahead of their first step: two for FP32 (the gather
+  // latency then spans two planes' B steps), one for 16-bit storage
+  constexpr int kAhead = sizeof(T) == 4 ? 2 : 1;
+  APending q0, q1;
+  q0.ok = false;
+  q1.ok = false;
+  // prologue: A planes zs-1 (if any) and zs, then the gathers of planes zs+1 (, zs+2)
   for (int32_t p = max(zs - 1, 0); p <= zs; ++p) {
-    if (a_ok) {
-      fused_a_gather<S>(fa.a, x0, y0, t, p, q);
-      fused_a_finish<S, SR>(fa.a, t, p, q, r);
-    }
+    fused_a_gather<S>(fa.a, x0, y0, t, p, q0);
+    if (a_ok) fused_a_finish<S, SR>(fa.a, t, p, q0, r);
     __syncthreads();  // the dn group of plane zs-1 is overwritten by plane zs
   }
-  if (a_ok && zs + 1 < nz) fused_a_gather<S>(fa.a, x0, y0, t, zs + 1, q);
+  // planes past the chunk (or the domain) are gathered too, clamped, and unused
+  fused_a_gather<S>(fa.a, x0, y0, t, min(zs + 1, nz - 1), q0);
+  if (kAhead == 2) fused_a_gather<S>(fa.a, x0, y0, t, min(zs + 2, nz - 1), q1);
 
   const int bx = t % kFTX, by = t / kFTX;
   const int32_t gx = x0 + bx, gy = y0 + by;
@@ -449,11 +460,15 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
   // local A coordinates of the (clamped) source columns
   const int lxm = gx == 0 ? bx + 1 : bx, lxp = gx == nx - 1 ? bx + 1 : bx + 2;
   const int lym = gy == 0 ? by + 1 : by, lyp = gy == ny - 1 ? by + 1 : by + 2;
-  for (int32_t z = zs; z < ze; ++z) {
-    if (a_ok && z + 1 < nz) {
-      fused_a_finish<S, SR>(fa.a, t, z + 1, q, r);
-      if (z + 1 < ze && z + 2 < nz) fused_a_gather<S>(fa.a, x0, y0, t, z + 2, q);
-    }
+  const uint32_t cxy = uint32_t(gy) * uint32_t(nx) + uint32_t(gx), pl = uint32_t(nx) * uint32_t(ny);
+  const int bc = (by + 1) * kFAX + bx + 1;  // the B cell among the A cells
+  // B plane z; q holds the gathered A plane z+1 and receives A plane z+1+kAhead.
+  // The B step issues no global load: its flag byte comes from the A step's
+  // copy in LDS, so no wait on a global load can hold back the A gathers
+  // still in flight (the counter of vector memory operations is in order).
+  auto plane = [&](int32_t z, APending& q) {
+    if (a_ok && z + 1 < nz) fused_a_finish<S, SR>(fa.a, t, z + 1, q, r);
+    fused_a_gather<S>(fa.a, x0, y0, t, min(z + 1 + kAhead, nz - 1), q);
     __syncthreads();
     if (b_ok) {
       // c_z = +1 pulls from plane z-1 (clamped to z), c_z = -1 from z+1 (the
@@ -473,15 +488,27 @@ __global__ __launch_bounds__(kFBlock) void k_fused2(const FusedArgs fa) {
       k.x = gx;
       k.y = gy;
       k.z = z;
-      k.c = (uint32_t(z) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx);
+      k.c = uint32_t(z) * pl + cxy;
       k.l = k.c;
-      const uint32_t fl = fa.a.sflag[k.c];
+      const uint32_t fl = r.fl[z % 2][bc];
       if (!(fl & kLoad)) clear(f);
       update_cell(fl, fa.a, k.c, f);
       if (fl & kStore)  // the second step produces state a.state + 1
         store_plain_all<S, SR>(fa.a, k, fa.a.dst, f, make_encoder<S, SR>(fa.a, k.c, fa.a.state + 1));
     }
     __syncthreads();
+  };
+  int32_t z = zs;
+  if constexpr (kAhead == 2) {
+    // both halves on every trip (an odd last plane follows the loop): the
+    // compiler then counts the other set's gather as in flight at every wait
+    for (; z + 1 < ze; z += 2) {
+      plane(z, q0);
+      plane(z + 1, q1);
+    }
+    if (z < ze) plane(z, q0);
+  } else {
+    for (; z < ze; ++z) plane(z, q0);
   }
 }
 
