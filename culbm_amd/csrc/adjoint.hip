@@ -77,29 +77,51 @@ __device__ __forceinline__ void adj_gather_consumers(const AdjArgs& j, const Cel
     lo[i] = (o.vx[cx + 1] && o.vy[cy + 1] && o.vz[cz + 1]) ? v : 0.0f;
   }
   if (!o.vx[0] || !o.vx[2]) {  // x face: the cell itself consumes along x
+    // every lane of the wave loads (all addresses are in the domain) and keeps
+    // the value where it applies: one wait for the batch instead of one
+    // round trip per conditional load
+    float e[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      const int cy = cy_of(i), cz = cz_of(i);
+      if (cx_of(i) != 0) e[i] = ld_at<float>(base + i * dsb, lb + uint32_t(o.oy[cy + 1] + o.oz[cz + 1]));
+    }
 #pragma unroll
     for (int i = 0; i < 27; ++i) {
       const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
       if (cx == 0) continue;
-      if (!o.vx[1 - cx] && o.vy[cy + 1] && o.vz[cz + 1])  // x - c_x leaves the domain
-        lo[i] += ld_at<float>(base + i * dsb, lb + uint32_t(o.oy[cy + 1] + o.oz[cz + 1]));
+      if (!o.vx[1 - cx] && o.vy[cy + 1] && o.vz[cz + 1]) lo[i] += e[i];  // x - c_x leaves the domain
     }
   }
   if (!(o.vy[0] && o.vy[2] && o.vz[0] && o.vz[2])) {
     // y or z face (whole rows and planes, so interior waves skip this): the
     // consumers clamped on a subset of the axes that contains y or z
+    // in three batches of nine directions (one c_z each), loads first
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-      const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
-      const bool sx = cx != 0 && !o.vx[1 - cx], sy = cy != 0 && !o.vy[1 - cy], sz = cz != 0 && !o.vz[1 - cz];
+    for (int g = 0; g < 3; ++g) {
+      float e[9][6];
 #pragma unroll
-      for (int m = 2; m < 8; ++m) {  // bit a of m: the consumer is the cell itself along axis a
-        if (((m & 1) && cx == 0) || ((m & 2) && cy == 0) || ((m & 4) && cz == 0)) continue;
-        const bool ok = ((m & 1) ? sx : o.vx[cx + 1]) && ((m & 2) ? sy : o.vy[cy + 1]) &&
-                        ((m & 4) ? sz : o.vz[cz + 1]);
-        const int32_t off = ((m & 1) ? 0 : o.ox[cx + 1]) + ((m & 2) ? 0 : o.oy[cy + 1]) +
-                            ((m & 4) ? 0 : o.oz[cz + 1]);
-        if (ok) lo[i] += ld_at<float>(base + i * dsb, lb + uint32_t(off));
+      for (int i = 9 * g; i < 9 * g + 9; ++i) {
+        const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
+#pragma unroll
+        for (int m = 2; m < 8; ++m) {  // bit a of m: the consumer is the cell itself along axis a
+          if (((m & 1) && cx == 0) || ((m & 2) && cy == 0) || ((m & 4) && cz == 0)) continue;
+          const int32_t off = ((m & 1) ? 0 : o.ox[cx + 1]) + ((m & 2) ? 0 : o.oy[cy + 1]) +
+                              ((m & 4) ? 0 : o.oz[cz + 1]);
+          e[i - 9 * g][m - 2] = ld_at<float>(base + i * dsb, lb + uint32_t(off));
+        }
+      }
+#pragma unroll
+      for (int i = 9 * g; i < 9 * g + 9; ++i) {
+        const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
+        const bool sx = cx != 0 && !o.vx[1 - cx], sy = cy != 0 && !o.vy[1 - cy], sz = cz != 0 && !o.vz[1 - cz];
+#pragma unroll
+        for (int m = 2; m < 8; ++m) {
+          if (((m & 1) && cx == 0) || ((m & 2) && cy == 0) || ((m & 4) && cz == 0)) continue;
+          const bool ok = ((m & 1) ? sx : o.vx[cx + 1]) && ((m & 2) ? sy : o.vy[cy + 1]) &&
+                          ((m & 4) ? sz : o.vz[cz + 1]);
+          if (ok) lo[i] += e[i - 9 * g][m - 2];
+        }
       }
     }
   }
@@ -215,7 +237,10 @@ __device__ __forceinline__ double block_sum(double v) {
 // One reverse step: mu^t from mu^{t+1} and f^t.  Both gathers are in bounds for every cell, so they are issued
 // before the flag arrives.  With wobj the objective's term is added to mu^t.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_adjoint(const AdjArgs j) {
+// Four waves per SIMD: the batched face loads would otherwise take the register
+// count past 128.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
+    const AdjArgs j) {
   const StepArgs& a = j.a;
   const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
   float dtau = 0.0f;
