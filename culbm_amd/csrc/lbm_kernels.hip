@@ -221,52 +221,74 @@ struct TwistLoad {
 };
 
 // Stores of a step of parity P (1 = O: slot 26-i, 0 = E: slot i) at x + p(c_i).
+//
+// Ghost copies of a face cell: a consumer is clamped onto this cell on axis a
+// when x_a - c_a leaves the domain (c_a = +1 on the low face, -1 on the high
+// face); its load slot on that axis is at p(-c_a) instead of p(c_a).  The
+// copies for consumers clamped along x alone (the frequent case: two cells of
+// every row) are stored next to each regular store; the rest (y or z faces:
+// whole rows and planes) by TwistGhostYZ afterwards, from the values just
+// stored, so that no population has to stay in a register for them.
 template <class S, int P, bool SR>
 struct TwistStore {
   char* base;
   int64_t dsb;
   TwistRows r;
   Encoder<S, SR> e;
+  uint32_t fx;  // bit 0: x = 0; bit 1: x = nx-1
   template <int I>
   __device__ __forceinline__ void run(const float* f) const {
+    using T = typename S::T;
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? 26 - I : I;
-    constexpr uint32_t ox = cx == 1 ? sizeof(typename S::T) : 0;
-    st_at<typename S::T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, e.template get<I>(f[I]));
+    constexpr uint32_t ox = cx == 1 ? sizeof(T) : 0;
+    const T v = e.template get<I>(f[I]);
+    st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, v);
+    if constexpr (cx != 0) {
+      constexpr uint32_t qx = cx == -1 ? sizeof(T) : 0;
+      if (cx == 1 ? (fx & 1u) : (fx >> 1)) st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + qx, v);
+    }
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
 
-// Ghost copies of a face cell: a consumer is clamped onto this cell on axis a
-// when x_a - c_a leaves the domain (c_a = +1 on the low face, -1 on the high
-// face); its load slot on that axis is at p(-c_a) instead of p(c_a).
-template <class S, int P, bool SR>
-struct TwistGhost {
+// Ghost copies that involve a y or z face (clamped along y and/or z, with or
+// without x).  The values are read back from the regular slots this cell has
+// just written (its own stores, so the same bits), all loads first.
+template <class S, int P>
+struct TwistGhostYZ {
+  using T = typename S::T;
   char* base;
   int64_t dsb;
   TwistRows r;
-  Encoder<S, SR> e;
   uint32_t fx, fy, fz;  // bit 0: coordinate 0 on the axis; bit 1: coordinate n-1
-  template <int I>
-  __device__ __forceinline__ void run(const float* f) const {
+  template <int I, int E>
+  __device__ __forceinline__ void load_range(T* v) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? 26 - I : I;
-    constexpr int32_t B = sizeof(typename S::T);
-    const uint32_t gx = cx == 1 ? (fx & 1u) : (cx == -1 ? (fx >> 1) : 0u);
-    const uint32_t gy = cy == 1 ? (fy & 1u) : (cy == -1 ? (fy >> 1) : 0u);
-    const uint32_t gz = cz == 1 ? (fz & 1u) : (cz == -1 ? (fz >> 1) : 0u);
-    const uint32_t gm = gx | (gy << 1) | (gz << 2);
-    if (gm) {
-      const typename S::T v = e.template get<I>(f[I]);  // the value of the regular store
+    constexpr uint32_t ox = cx == 1 ? sizeof(T) : 0;
+    if constexpr (cy != 0 || cz != 0) v[I] = ld_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox);
+    if constexpr (I + 1 < E) load_range<I + 1, E>(v);
+  }
+  template <int I, int E>
+  __device__ __forceinline__ void store_range(const T* v) const {
+    constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
+    if constexpr (cy != 0 || cz != 0) {
+      constexpr int slot = P == 1 ? 26 - I : I;
+      constexpr int32_t B = sizeof(T);
+      const uint32_t gx = cx == 1 ? (fx & 1u) : (cx == -1 ? (fx >> 1) : 0u);
+      const uint32_t gy = cy == 1 ? (fy & 1u) : (cy == -1 ? (fy >> 1) : 0u);
+      const uint32_t gz = cz == 1 ? (fz & 1u) : (cz == -1 ? (fz >> 1) : 0u);
+      const uint32_t gm = gx | (gy << 1) | (gz << 2);
       char* p = base + slot * dsb;
       const int32_t sx = cx == 1 ? B : 0, sy = cy == 1 ? r.row : 0, sz = cz == 1 ? r.plane : 0;
       const int32_t qx = cx == -1 ? B : 0, qy = cy == -1 ? r.row : 0, qz = cz == -1 ? r.plane : 0;
-      for (uint32_t m = 1; m < 8; ++m)
+      for (uint32_t m = 2; m < 8; ++m)
         if ((m & ~gm) == 0)
-          st_at<typename S::T>(p, r.lb + uint32_t(((m & 1u) ? qx : sx) + ((m & 2u) ? qy : sy) +
-                                                  ((m & 4u) ? qz : sz)), v);
+          st_at<T>(p, r.lb + uint32_t(((m & 1u) ? qx : sx) + ((m & 2u) ? qy : sy) + ((m & 4u) ? qz : sz)),
+                   v[I]);
     }
-    if constexpr (I < 26) run<I + 1>(f);
+    if constexpr (I + 1 < E) store_range<I + 1, E>(v);
   }
 };
 
@@ -289,20 +311,26 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   t.dsb = a.dstride * int64_t(sizeof(typename S::T));
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
   t.e = e;
+  t.fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
   t.template run<0>(f);
-  const uint32_t fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
   const uint32_t fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
   const uint32_t fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
-  if (fx | fy | fz) {  // one branch per wave; interior waves skip it entirely
-    TwistGhost<S, P, SR> g;
+  if (fy | fz) {  // waves on a y or z face only
+    TwistGhostYZ<S, P> g;
     g.base = t.base;
     g.dsb = t.dsb;
     g.r = t.r;
-    g.e = e;
-    g.fx = fx;
+    g.fx = t.fx;
     g.fy = fy;
     g.fz = fz;
-    g.template run<0>(f);
+    typename S::T v[27];
+    // three batches of nine directions (one c_z each), loads first
+    g.template load_range<0, 9>(v);
+    g.template store_range<0, 9>(v);
+    g.template load_range<9, 18>(v);
+    g.template store_range<9, 18>(v);
+    g.template load_range<18, 27>(v);
+    g.template store_range<18, 27>(v);
   }
 }
 
