@@ -231,7 +231,7 @@ def precision_report(n=512, steps=500):
     return out
 
 
-def cpu_baseline(state, budget_s=15.0):
+def cpu_baseline(state, budget_s=12.0):
     """FP32 OpenMP restatement of the plain pull step on the same workload."""
     os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime loads
     from oracle import pyoracle
@@ -242,7 +242,7 @@ def cpu_baseline(state, budget_s=15.0):
     t0 = time.perf_counter()
     sim.step(1)
     t1 = time.perf_counter() - t0
-    n = int(max(2, min(10, budget_s / max(t1, 1e-3))))
+    n = int(max(2, min(100, budget_s / max(t1, 1e-3))))  # about budget_s of CPU time
     t0 = time.perf_counter()
     sim.step(n)
     dt = time.perf_counter() - t0
