@@ -7,6 +7,10 @@ MI355X counterparts of the reference's microbenchmarks:
   * gamma: partial-sector store penalty (analysis/gamma/measure.cu:14-69); here
     a copy whose every store is shifted by one element (each wave's store
     straddles one extra 128-byte line) against the aligned copy.
+  * the ceiling of the population access pattern itself: 27 arrays read and
+    written once per cell (pop27), with the pull gather's x offsets
+    (pop27_shifted).  No step kernel can stream faster than this without
+    cache reuse.
 
     python -m culbm_amd.calibrate [--quick] [--device N]
 
@@ -27,7 +31,7 @@ MB = 1 << 20
 def measure(device: int = 0, quick: bool = False) -> dict:
     sizes = (64, 2048) if quick else (16, 64, 128, 256, 512, 1024, 2048, 4096)
     res = {"sizes_mb_per_buffer": list(sizes)}
-    for mode in ("copy", "read", "write", "copy_scalar", "copy_shifted"):
+    for mode in ("copy", "read", "write", "copy_scalar", "copy_shifted", "pop27", "pop27_shifted"):
         res[mode] = [round(probe_bandwidth(mode, s * MB, iters=10 if s < 1024 else 4, device=device), 1)
                      for s in sizes]
     small = sizes.index(64)
@@ -35,6 +39,8 @@ def measure(device: int = 0, quick: bool = False) -> dict:
     res["lambda_hbm_over_mall"] = round(res["copy"][big] / res["copy"][small], 4)
     res["gamma_shifted_over_aligned_hbm"] = round(res["copy_shifted"][big] / res["copy_scalar"][big], 4)
     res["gamma_shifted_over_aligned_mall"] = round(res["copy_shifted"][small] / res["copy_scalar"][small], 4)
+    # FP32 MLUPS that the pattern's HBM-regime rate allows at 216 B per update
+    res["pop27_mlups_ceiling_fp32"] = round(res["pop27_shifted"][big] * 1e3 / 216, 0)
     return res
 
 

@@ -1022,7 +1022,7 @@ int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, in
 }
 
 int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs) {
-  if (bytes < 4096 || bytes > (int64_t(64) << 30) || iters < 1 || mode < 0 || mode > 4)
+  if (bytes < 4096 || bytes > (int64_t(64) << 30) || iters < 1 || mode < 0 || mode > 6)
     return fail(nullptr, CULBM_ERR_INVALID, "bad bandwidth probe arguments");
   if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CULBM_ERR_HIP, "hipSetDevice failed");
   const uint64_t n4 = uint64_t(bytes) / 16;
@@ -1034,18 +1034,23 @@ int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t i
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(nullptr, CULBM_ERR_HIP, "event creation failed");
   if (!rc) {
-    hipDeviceProp_t prop;
-    (void)hipGetDeviceProperties(&prop, device);
-    const uint32_t nblk = uint32_t(prop.multiProcessorCount) * 16;
-    hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(a), n4);
-    hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4);
+    // one element (or one cell) per thread
+    const uint32_t nb4 = uint32_t((n4 + kBlock - 1) / kBlock), nb1 = uint32_t((4 * n4 + kBlock - 1) / kBlock);
+    // modes 5, 6: 27 arrays per buffer, each starting on a 1 KiB boundary like
+    // the population layout's direction stride
+    const uint64_t cells = 4 * n4 / 27 / 256 * 256;
+    const uint32_t nbc = uint32_t((cells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(a), n4);
+    hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4);
     auto launch = [&]() {
       switch (mode) {
-        case 0: hipLaunchKernelGGL(k_bw_copy, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float4*>(b), n4); break;
-        case 1: hipLaunchKernelGGL(k_bw_read, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float*>(b), n4); break;
-        case 2: hipLaunchKernelGGL(k_bw_write, dim3(nblk), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4); break;
-        case 3: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 0u); break;
-        default: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nblk), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 1u); break;
+        case 0: hipLaunchKernelGGL(k_bw_copy, dim3(nb4), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float4*>(b), n4); break;
+        case 1: hipLaunchKernelGGL(k_bw_read, dim3(nb4), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float*>(b), n4); break;
+        case 2: hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4); break;
+        case 3: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 0u); break;
+        case 4: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 1u); break;
+        case 5: hipLaunchKernelGGL(k_bw_pop27, dim3(nbc), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), cells, 0u); break;
+        default: hipLaunchKernelGGL(k_bw_pop27, dim3(nbc), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), cells, 1u); break;
       }
     };
     launch();  // warm-up
@@ -1055,7 +1060,8 @@ int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t i
     if (hipEventSynchronize(e1) != hipSuccess) rc = fail(nullptr, CULBM_ERR_HIP, "probe kernel failed");
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    const double moved = double(n4) * 16.0 * (mode == 0 || mode >= 3 ? 2.0 : 1.0) * iters;
+    const double moved = (mode >= 5 ? double(cells) * 27.0 * 4.0 : double(n4) * 16.0) *
+                         (mode == 1 || mode == 2 ? 1.0 : 2.0) * iters;
     if (gbs && ms > 0.0f) *gbs = float(moved / (double(ms) * 1e-3) / 1e9);
   }
   if (e0) (void)hipEventDestroy(e0);

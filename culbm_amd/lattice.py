@@ -279,7 +279,8 @@ def probe_codec(x: np.ndarray, storage: str, scale: float = 1.0, device: int = 0
     return enc, dec
 
 
-BW_MODES = {"copy": 0, "read": 1, "write": 2, "copy_scalar": 3, "copy_shifted": 4}
+BW_MODES = {"copy": 0, "read": 1, "write": 2, "copy_scalar": 3, "copy_shifted": 4, "pop27": 5,
+            "pop27_shifted": 6}
 
 
 def probe_bandwidth(mode: str, nbytes: int, iters: int = 20, device: int = 0) -> float:

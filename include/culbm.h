@@ -250,9 +250,12 @@ int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, in
                         int32_t max_starts);
 
 /* Bandwidth calibration (counterpart of the reference's analysis/lambda and
- * analysis/gamma probes): mode 0 copy, 1 read, 2 write (16-byte accesses),
- * 3 scalar copy, 4 scalar copy with every store shifted by one element.
- * `bytes` per buffer; *gbs = bytes moved / time. */
+ * analysis/gamma probes), one element per thread: mode 0 copy, 1 read,
+ * 2 write (16-byte accesses), 3 scalar copy, 4 scalar copy with every store
+ * shifted by one element, 5 the population access pattern (27 arrays, one
+ * cell per thread, each value read and written once), 6 the same with the
+ * pull gather's +-1 x offsets on 18 of the 27 loads.  `bytes` per buffer;
+ * *gbs = bytes moved / time. */
 int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs);
 
 #ifdef __cplusplus

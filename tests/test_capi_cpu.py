@@ -127,3 +127,11 @@ class TestCli:
     def test_missing_flag_file(self, tmp_path):
         r = run_cli("--initStateFolder", str(tmp_path), "--domainDim", "[8,8,8]")
         assert r.returncode == 1 and f"File: {tmp_path}/flag.dat doesn't exist!" in r.stderr
+
+
+@pytest.mark.parametrize("mode,nbytes,iters", [(7, 1 << 20, 1), (-1, 1 << 20, 1), (0, 100, 1), (5, 1 << 20, 0),
+                                               (6, (64 << 30) + 16, 1)])
+def test_bandwidth_probe_rejects_bad_arguments_before_touching_a_device(mode, nbytes, iters):
+    g = ctypes.c_float()
+    rc = _lib.lib().culbm_probe_bandwidth(0, mode, nbytes, iters, ctypes.byref(g))
+    assert rc == 1  # CULBM_ERR_INVALID
