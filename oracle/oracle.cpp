@@ -21,8 +21,27 @@
 //     with -ffp-contract=off, so results are reproducible bit for bit.  The
 //     FP32 lattice step is OpenMP-parallel over (z, y) rows and doubles as
 //     the CPU baseline.
-// 16-bit shifted storage (FP16 / BF16 with a power-of-two scale) has no
-// reference counterpart; the functions here define it.
+// 16-bit shifted storage (FP16 / BF16 with a power-of-two scale), its
+// stochastic rounding and the discrete adjoint have no reference
+// counterpart; the functions here define them.
+//
+// What pins each part (the tests under tests/):
+//   - equilibrium / moments / collision: golden vectors from the reference
+//     headers compiled on the host (bit-exact), the closed forms and
+//     conservation laws in FP64, feq as a fixed point;
+//   - pull step, clamp, bounce-back, dump: a brute-force pure-Python step on
+//     tiny grids, a uniform equilibrium that stays steady under the face
+//     clamp, and cell-id tags streamed through every GPU policy (the step's
+//     own source, kernel.cu, does not build here);
+//   - blocking math: exhaustive brute force over small (N, B, I) and the
+//     reference header's functions through oracle/_ref;
+//   - 16-bit codecs: round-to-nearest-even against numpy's float16 and the
+//     bfloat16 bit truncation rule; stochastic rounding: each result is one
+//     of the two neighbouring representable values, its mean converges to
+//     the input (unbiased), and it is a function of (cell, direction, state);
+//   - adjoint: central finite differences of the same FP64 forward run
+//     (rho/u of EQU cells, invTau, f^0), and the checkpointed parallel form
+//     against the serial one bit for bit.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
