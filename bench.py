@@ -42,6 +42,8 @@ FORMS = {
     "fp16_plain_pull": (0, 0, "fp16", "FP16-shifted plain pull"),
     "fp32_mixed": (0, 2, "fp32", "FP32 pull + mixed blocking (two steps per launch, LDS)"),
     "fp16_mixed": (0, 2, "fp16", "FP16-shifted pull + mixed blocking (two steps per launch, LDS)"),
+    "fp16_sr_inplace_blocked": (1, 3, "fp16", "FP16-shifted in-place + dynamic blocking, stochastic rounding",
+                                "stochastic"),
 }
 HEADLINE = "fp32_inplace_blocked"
 
@@ -59,9 +61,10 @@ def _sync():
 def run_form(state, name, steps, warmup, inner_loop):
     from culbm_amd import Lattice, LatticeConfig
 
-    sp, op, storage, _ = FORMS[name]
+    sp, op, storage = FORMS[name][:3]
+    rounding = FORMS[name][4] if len(FORMS[name]) > 4 else "nearest"
     lat = Lattice(LatticeConfig(domain=DOMAIN, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
-                                inner_loop=inner_loop, storage=storage))
+                                inner_loop=inner_loop, storage=storage, rounding=rounding))
     lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
     lat.init_ddf("rest")
     lat.step(warmup)
@@ -331,6 +334,7 @@ def main(argv=None) -> int:
             "stream_policy": FORMS[ns.form][0],
             "opt_policy": FORMS[ns.form][1],
             "storage": FORMS[ns.form][2],
+            "rounding": FORMS[ns.form][4] if len(FORMS[ns.form]) > 4 else "nearest",
             "inner_loop": ns.inner_loop,
             "wave_depth": head["wave_depth"],
         },
