@@ -266,6 +266,28 @@ def test_full_size_blocked_equals_plain():
     assert_bitexact(out[0][1], out[1][1], "full-size populations")
 
 
+def test_precision_workload_512_blocked_equals_plain():
+    """512^3 (134M cells, the largest configuration), FP16 shifted storage with the
+    flow started from feq: in-place + wavefront blocking against plain pull over a
+    full pass and a partial one.  Every cell's dumped rho/u must agree bit for bit
+    and the FP64 sums of the stored populations to 1e-12 (reading back 14.5 GB of
+    populations is left out)."""
+    st = cases.config_precision(512)
+    steps = 7
+    out = []
+    for sp, op in ((0, 0), (1, 3)):
+        lat = make(st, init="feq", inv_tau=0.5, stream_policy=sp, opt_policy=op, inner_loop=5,
+                   storage="fp16")
+        lat.step(steps)
+        lat.dump()
+        out.append((lat.macros(), lat.moment_sums()))
+        lat.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert_bitexact(a, b, "512^3 macros")
+    for a, b in zip(out[0][1], out[1][1]):
+        assert abs(a - b) <= 1e-12 * max(abs(a), 1.0), (out[0][1], out[1][1])
+
+
 def test_mixed_with_inactive_cells_and_midrun_flag_change():
     """Cells without STORE keep their populations in both buffers; the fused
     two-step kernel reads them from src, plain pull alternates buffers: both
