@@ -135,6 +135,16 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
   const int64_t padded = round_up(int64_t(c->nx) + 1, 64) * (int64_t(c->ny) + 1) * (int64_t(c->nz) + 1);
   if (n > (int64_t(1) << 31) - 1 || padded > (int64_t(1) << 32) - 1)
     return bad(CULBM_ERR_INVALID, "domain of %lld cells exceeds the 32-bit cell index", (long long)n);
+  {
+    // the kernels address one direction array with 32-bit per-lane byte offsets
+    const int64_t bpp = (c->storage == CULBM_STORAGE_FP16 || c->storage == CULBM_STORAGE_BF16) ? 2 : 4;
+    const int64_t per_dir = (c->stream_policy == CULBM_STREAM_INPLACE ? padded : n) * bpp;
+    if (per_dir > (int64_t(1) << 32))  // the last element starts at per_dir - bpp
+      return bad(CULBM_ERR_INVALID,
+                 "domain too large: one direction array takes %lld bytes, above the 4 GiB the "
+                 "kernels' 32-bit offsets reach (use 16-bit storage or a smaller domain)",
+                 (long long)per_dir);
+  }
   if (!std::isfinite(c->inv_tau)) return bad(CULBM_ERR_INVALID, "invTau must be finite");
   if (c->stream_policy != CULBM_STREAM_PULL && c->stream_policy != CULBM_STREAM_INPLACE)
     return bad(CULBM_ERR_INVALID, "streamPolicy must be 0 (pull) or 1 (in-place), got %d",

@@ -62,10 +62,21 @@ def test_defaults_are_the_reference_cli_defaults():
      "<Domain Dimension, Blocking Dimension and InnerLoop> don't match!"),
     (dict(inv_tau=float("nan")), "finite"),
     (dict(storage="fp32", rounding="stochastic"), "stochastic rounding applies to FP16 or BF16"),
+    # one FP32 direction array above 4 GiB: padded in place (1088 x 1025 x 1025 cells), dense in pull
+    (dict(domain=(1024, 1024, 1024), stream_policy=1), "32-bit offsets"),
+    (dict(domain=(1100, 1100, 1100)), "32-bit offsets"),
 ])
 def test_validation_errors(kw, msg):
     err = validate(LatticeConfig(**kw))
     assert err is not None and msg in err
+
+
+def test_largest_domains_that_fit_the_32_bit_offsets_are_accepted():
+    # FP32 pull at 1024^3: 2^30 cells x 4 B, exactly 4 GiB per direction, fits (the last
+    # element starts at 2^32 - 4); one more plane does not; FP16 in place at 1024^3 fits
+    assert validate(LatticeConfig(domain=(1024, 1024, 1024))) is None
+    assert "32-bit offsets" in validate(LatticeConfig(domain=(1024, 1024, 1025)))
+    assert validate(LatticeConfig(domain=(1024, 1024, 1024), stream_policy=1, storage="fp16")) is None
 
 
 def test_valid_configs():
