@@ -248,6 +248,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     float h[27], lam[27];
     make_gather<S>(a, k, a.src).template run<0>(h);
     adj_gather_consumers(j, k, j.mu_in, lam);
+    // x-face cells (the reference convention's EQU cells) fetch their EQU
+    // inputs and gradient slot with the rest of the batch
+    EquIn q = equ_prefetch(a, k.c, k.x);
+    const uint32_t aux_q = j.aux[q.ok ? k.c : 0u];
     const uint32_t fl = a.sflag[k.c];
     // objective term w (c_x - u_x) / rho of the pulled populations, added to mu^t last
     float wq = 0.0f, wux = 0.0f;
@@ -261,16 +265,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
     if (fl & kStore) {
       const bool bb = fl & kBounceBack;
+      if ((fl & kEqu) && !q.ok) {  // an EQU cell off the x faces
+        q.rho = a.rho[k.c];
+        q.vx = a.vx[k.c];
+        q.vy = a.vy[k.c];
+        q.vz = a.vz[k.c];
+      }
       if (fl & kEqu)
-        equilibrium(a.rho[k.c], a.vx[k.c], a.vy[k.c], a.vz[k.c], h);
+        equilibrium(q.rho, q.vx, q.vy, q.vz, h);
       else if (!(fl & kLoad))
         clear(h);
       if (fl & kCollide) dtau = adj_collide(a.inv_tau, h, lam, bb);
       if (bb) bounce_back(lam);
       if (fl & kEqu) {
         float g4[4];
-        adj_feq_inputs(a.rho[k.c], a.vx[k.c], a.vy[k.c], a.vz[k.c], lam, g4);
-        const uint32_t r = j.aux[k.c];
+        adj_feq_inputs(q.rho, q.vx, q.vy, q.vz, lam, g4);
+        const uint32_t r = q.ok ? aux_q : j.aux[k.c];
 #pragma unroll
         for (int q = 0; q < 4; ++q) j.grad4[q * j.n_equ + r] += double(g4[q]);
       }

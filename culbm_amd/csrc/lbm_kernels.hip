@@ -158,10 +158,45 @@ __device__ __forceinline__ void clear(float* f) {
   for (int i = 0; i < 27; ++i) f[i] = 0.0f;
 }
 
+// EQU inputs of a cell on an x face (x = 0 or nx-1), loaded in the same batch
+// as its populations.  An EQU cell's inputs are otherwise loaded only once its
+// flag has arrived: a second round trip, and, since the counter of vector
+// memory operations retires in order, a wait for every load still in flight.
+// The reference convention puts its EQU cells (inlet, equilibrium outlet) on
+// the x faces; an EQU cell anywhere else still loads its inputs on demand.
+struct EquIn {
+  float rho, vx, vy, vz;
+  bool ok;  // the values are loaded (x-face cell)
+};
+
+// Every lane loads (the others read cell 0: one cache line for the whole
+// launch): a branch around the loads would split the batch, and the compiler
+// then keeps 60 more registers live in the in-place kernel.
+__device__ __forceinline__ EquIn equ_prefetch(const StepArgs& a, uint32_t c, int32_t x) {
+  EquIn q;
+  q.ok = x == 0 || x == a.nx - 1;
+  const uint32_t cc = q.ok ? c : 0u;
+  q.rho = a.rho[cc];
+  q.vx = a.vx[cc];
+  q.vy = a.vy[cc];
+  q.vz = a.vz[cc];
+  return q;
+}
+
 // The per-cell update shared by every variant: LOAD -> EQU -> BB -> COLLIDE
 // (kernel.cu:2501-2631).  f holds the loaded values on entry.
-__device__ __forceinline__ void update_cell(uint32_t fl, const StepArgs& a, uint32_t c, float* f) {
-  if (fl & kEqu) equilibrium(a.rho[c], a.vx[c], a.vy[c], a.vz[c], f);
+__device__ __forceinline__ void update_cell(uint32_t fl, const StepArgs& a, uint32_t c, const EquIn& q,
+                                            float* f) {
+  if (fl & kEqu) {
+    float rho = q.rho, vx = q.vx, vy = q.vy, vz = q.vz;
+    if (!q.ok) {  // an EQU cell off the x faces
+      rho = a.rho[c];
+      vx = a.vx[c];
+      vy = a.vy[c];
+      vz = a.vz[c];
+    }
+    equilibrium(rho, vx, vy, vz, f);
+  }
   if (fl & kBounceBack) bounce_back(f);
   if (fl & kCollide) collide(a.inv_tau, a.omt, f);
 }
@@ -174,9 +209,10 @@ __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   // the flag arrives instead of waiting on it.
   float f[27];
   make_gather<S>(a, k, a.src).template run<0>(f);
+  const EquIn q = equ_prefetch(a, k.c, k.x);
   const uint32_t fl = a.sflag[k.c];
   if (!(fl & kLoad)) clear(f);
-  update_cell(fl, a, k.c, f);
+  update_cell(fl, a, k.c, q, f);
   if (fl & kStore) store_plain_all<S, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
 
@@ -342,9 +378,10 @@ __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   // before the flag arrives
   float f[27];
   twist_load_all<S, P>(a, k, a.dst, f);
+  const EquIn q = equ_prefetch(a, k.c, k.x);
   const uint32_t fl = a.sflag[k.c];
   if (!(fl & kLoad)) clear(f);
-  update_cell(fl, a, k.c, f);
+  update_cell(fl, a, k.c, q, f);
   if (fl & kStore) twist_store_all<S, P, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
 
@@ -384,12 +421,14 @@ struct FusedRing {
   T mid[2][9][kFA]; // directions 9..17 (c_z = 0), slot = plane % 2
   T dn[9][kFA];     // directions 0..8 (c_z = -1), the latest plane
   uint8_t fl[2][kFA];  // step bits of the A cells, slot = plane % 2 (the B step's flags)
+  float eq[2][2][kFTY][4];  // EQU inputs of the B cells on the low / high x face, slot = plane % 2
 };
 
 // Pulled (not yet updated) populations of one A cell, gathered a plane ahead
 // so the HBM latency overlaps the previous plane's second step.
 struct APending {
   float f[27];
+  EquIn e;
   uint8_t fl;  // kept 8-bit: a widened copy across the loop would wait for every load
   uint32_t c;
   bool ok;
@@ -412,6 +451,7 @@ __device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, in
   k.l = k.c;
   q.c = k.c;
   make_gather<S>(a, k, a.src).template run<0>(q.f);
+  q.e = equ_prefetch(a, k.c, k.x);
   q.fl = a.sflag[k.c];
 }
 
@@ -429,7 +469,7 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   using T = typename S::T;
   if (!q.ok) return;
   if (!(q.fl & kLoad)) clear(q.f);
-  update_cell(q.fl, a, q.c, q.f);
+  update_cell(q.fl, a, q.c, q.e, q.f);
   T v[27];
   if (q.fl & kStore) {
     encode_all<S, SR, 0>(q.f, v, make_encoder<S, SR>(a, q.c, a.state));
@@ -441,6 +481,15 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   }
   const int su = p % 3, sm = p % 2;
   r.fl[sm][ac] = uint8_t(q.fl);
+  // EQU inputs of the x-face cells among the B cells, for the second step
+  const int ly = ac / kFAX - 1;
+  if (q.e.ok && ly >= 0 && ly < kFTY) {
+    float* d = r.eq[sm][q.c % uint32_t(a.nx) == 0 ? 0 : 1][ly];
+    d[0] = q.e.rho;
+    d[1] = q.e.vx;
+    d[2] = q.e.vy;
+    d[3] = q.e.vz;
+  }
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     r.dn[j][ac] = v[j];
@@ -519,8 +568,15 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
       k.c = uint32_t(z) * pl + cxy;
       k.l = k.c;
       const uint32_t fl = r.fl[z % 2][bc];
+      EquIn e;
+      e.ok = gx == 0 || gx == nx - 1;
+      const float* q4 = r.eq[z % 2][gx == 0 ? 0 : 1][by];
+      e.rho = q4[0];
+      e.vx = q4[1];
+      e.vy = q4[2];
+      e.vz = q4[3];
       if (!(fl & kLoad)) clear(f);
-      update_cell(fl, fa.a, k.c, f);
+      update_cell(fl, fa.a, k.c, e, f);
       if (fl & kStore)  // the second step produces state a.state + 1
         store_plain_all<S, SR>(fa.a, k, fa.a.dst, f, make_encoder<S, SR>(fa.a, k.c, fa.a.state + 1));
     }
