@@ -428,7 +428,6 @@ struct FusedRing {
 // so the HBM latency overlaps the previous plane's second step.
 struct APending {
   float f[27];
-  EquIn e;
   uint8_t fl;  // kept 8-bit: a widened copy across the loop would wait for every load
   uint32_t c;
   bool ok;
@@ -451,7 +450,6 @@ __device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, in
   k.l = k.c;
   q.c = k.c;
   make_gather<S>(a, k, a.src).template run<0>(q.f);
-  q.e = equ_prefetch(a, k.c, k.x);
   q.fl = a.sflag[k.c];
 }
 
@@ -468,8 +466,22 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
                                                FusedRing<typename S::T>& r) {
   using T = typename S::T;
   if (!q.ok) return;
+  // An x-face EQU cell loads its inputs here, once its flag is known (carrying
+  // them in the two pending sets would spill registers in FP32), and passes
+  // them to the second step through LDS, where a load would wait for the
+  // gathers in flight.
+  EquIn qe;
+  const uint32_t gx = q.c % uint32_t(a.nx);
+  qe.ok = (q.fl & kEqu) && (gx == 0 || gx == uint32_t(a.nx - 1));
+  qe.rho = qe.vx = qe.vy = qe.vz = 0.0f;
+  if (qe.ok) {
+    qe.rho = a.rho[q.c];
+    qe.vx = a.vx[q.c];
+    qe.vy = a.vy[q.c];
+    qe.vz = a.vz[q.c];
+  }
   if (!(q.fl & kLoad)) clear(q.f);
-  update_cell(q.fl, a, q.c, q.e, q.f);
+  update_cell(q.fl, a, q.c, qe, q.f);
   T v[27];
   if (q.fl & kStore) {
     encode_all<S, SR, 0>(q.f, v, make_encoder<S, SR>(a, q.c, a.state));
@@ -483,12 +495,12 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   r.fl[sm][ac] = uint8_t(q.fl);
   // EQU inputs of the x-face cells among the B cells, for the second step
   const int ly = ac / kFAX - 1;
-  if (q.e.ok && ly >= 0 && ly < kFTY) {
-    float* d = r.eq[sm][q.c % uint32_t(a.nx) == 0 ? 0 : 1][ly];
-    d[0] = q.e.rho;
-    d[1] = q.e.vx;
-    d[2] = q.e.vy;
-    d[3] = q.e.vz;
+  if (qe.ok && ly >= 0 && ly < kFTY) {
+    float* d = r.eq[sm][gx == 0 ? 0 : 1][ly];
+    d[0] = qe.rho;
+    d[1] = qe.vx;
+    d[2] = qe.vy;
+    d[3] = qe.vz;
   }
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
