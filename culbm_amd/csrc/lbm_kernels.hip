@@ -517,10 +517,31 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
   using T = typename S::T;
   __shared__ FusedRing<T> r;
   const uint32_t ntile = uint32_t(fa.tiles_x) * fa.tiles_y * fa.nchunk;
-  const uint32_t w = xcd_block(blockIdx.x, ntile);
-  const int32_t tx = int32_t(w % uint32_t(fa.tiles_x));
-  const int32_t ty = int32_t((w / uint32_t(fa.tiles_x)) % uint32_t(fa.tiles_y));
-  const int32_t zc = int32_t(w / (uint32_t(fa.tiles_x) * fa.tiles_y));
+  int32_t tx, ty, zc;
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the tiles of the first and last tile column (their x-face EQU
+    // cells make them the slowest) are dispatched first, spread over the XCDs
+    // round robin, so that they do not form the tail of the launch
+    const uint32_t w = blockIdx.x;
+    const uint32_t nside = fa.tiles_x > 1 ? 2u : 1u;
+    const uint32_t nface = nside * uint32_t(fa.tiles_y) * uint32_t(fa.nchunk);
+    if (w < nface) {
+      const uint32_t side = w % nside, rest = w / nside;
+      tx = side == 0 ? 0 : fa.tiles_x - 1;
+      ty = int32_t(rest % uint32_t(fa.tiles_y));
+      zc = int32_t(rest / uint32_t(fa.tiles_y));
+    } else {
+      const uint32_t i = w - nface, ni = uint32_t(fa.tiles_x - 2);
+      tx = 1 + int32_t(i % ni);
+      ty = int32_t((i / ni) % uint32_t(fa.tiles_y));
+      zc = int32_t(i / (ni * uint32_t(fa.tiles_y)));
+    }
+  } else {
+    const uint32_t w = xcd_block(blockIdx.x, ntile);
+    tx = int32_t(w % uint32_t(fa.tiles_x));
+    ty = int32_t((w / uint32_t(fa.tiles_x)) % uint32_t(fa.tiles_y));
+    zc = int32_t(w / (uint32_t(fa.tiles_x) * fa.tiles_y));
+  }
   const int32_t x0 = tx * kFTX, y0 = ty * kFTY;
   const int32_t nx = fa.a.nx, ny = fa.a.ny, nz = fa.a.nz;
   const int32_t zs = zc * fa.zchunk, ze = min(nz, zs + fa.zchunk);
