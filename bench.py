@@ -313,7 +313,7 @@ def main(argv=None) -> int:
         t = json.loads(tpath.read_text()).get(ns.form)
         if t:
             cells_per_launch = DOMAIN[0] * DOMAIN[1] * DOMAIN[2] / head["launches_per_step"]
-            traffic = round(t["bytes_per_cell"] * cells_per_launch)
+            traffic = round(t.get("bytes_per_cell", t.get("bytes_per_update", 0.0)) * cells_per_launch)
     line = {
         "metric": "MLUPS D3Q27 at 288x272x280 (% HBM roofline); adjoint/forward step ratio",
         "value": round(head["mlups"], 3),
