@@ -242,7 +242,7 @@ template <class S>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
     const AdjArgs j) {
   const StepArgs& a = j.a;
-  const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
+  const Cell k = locate(a, cell_block(a));
   float dtau = 0.0f;
   if (k.ok) {
     float h[27], lam[27];
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // the cells without STORE.
 __global__ __launch_bounds__(kBlock) void k_adjoint_final(const AdjArgs j, float* out) {
   const StepArgs& a = j.a;
-  const Cell k = locate(a, xcd_block(blockIdx.x, a.nblk));
+  const Cell k = locate(a, cell_block(a));
   if (!k.ok) return;
   float lam[27];
   adj_gather_consumers(j, k, j.mu_in, lam);

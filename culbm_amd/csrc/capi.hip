@@ -235,11 +235,21 @@ StepArgs base_args(const culbm_ctx* ctx) {
   return a;
 }
 
-void set_range(StepArgs& a, int32_t z0, int32_t z1) {
+// Workgroup order of a launch over planes [z0, z1) (see cell_block).  The
+// plane-range launches of the wavefront take the XCD-contiguous remap.  Launches
+// over the whole domain run in dispatch order, which is the faster one from 192^3
+// up in every form (FP32 pull +1.8 % at 192^3, +6.6 % at 288x272x280; FP16 pull
+// +11 %), except the gather kernels (pull, adjoint) below 2^22 cells, where each
+// XCD's L2 serving its own slab's neighbour planes wins (FP32 pull at 128^3: +5 %).
+constexpr uint32_t kRemapGatherCells = 1u << 22;
+
+void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather) {
   a.z0 = z0;
   a.z1 = z1;
   a.ncells = uint32_t(z1 - z0) * uint32_t(a.nx) * uint32_t(a.ny);
   a.nblk = (a.ncells + kBlock - 1) / kBlock;
+  const bool part = z0 > 0 || z1 < a.nz;
+  a.xcd = (part || (gather && a.ncells < kRemapGatherCells)) ? 1u : 0u;
 }
 
 // Event pair around one launch when profiling is on.
@@ -276,7 +286,7 @@ void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity) {
 
 template <class S>
 int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) {
-  set_range(a, z0, z1);
+  set_range(a, z0, z1, !ctx->inplace);
   if (a.ncells == 0) return CULBM_OK;
   if (int rc = prof_begin(ctx)) return rc;
   if (ctx->inplace) {
@@ -449,7 +459,7 @@ int steps_dispatch(culbm_ctx* ctx, int32_t nsteps) {
 
 StepArgs full_args(const culbm_ctx* ctx) {
   StepArgs a = base_args(ctx);
-  set_range(a, 0, ctx->cfg.nz);
+  set_range(a, 0, ctx->cfg.nz, false);
   return a;
 }
 
