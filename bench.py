@@ -40,6 +40,8 @@ FORMS = {
     "fp16_inplace_blocked": (1, 3, "fp16", "FP16-shifted in-place + dynamic blocking"),
     "bf16_inplace_blocked": (1, 3, "bf16", "BF16-shifted in-place + dynamic blocking"),
     "fp16_plain_pull": (0, 0, "fp16", "FP16-shifted plain pull"),
+    "fp32_pull_blocked": (0, 3, "fp32", "FP32 pull + dynamic (wavefront) temporal blocking"),
+    "fp16_pull_blocked": (0, 3, "fp16", "FP16-shifted pull + dynamic blocking"),
     "fp32_mixed": (0, 2, "fp32", "FP32 pull + mixed blocking (two steps per launch, LDS)"),
     "fp16_mixed": (0, 2, "fp16", "FP16-shifted pull + mixed blocking (two steps per launch, LDS)"),
     "fp16_sr_inplace_blocked": (1, 3, "fp16", "FP16-shifted in-place + dynamic blocking, stochastic rounding",
