@@ -236,13 +236,13 @@ StepArgs base_args(const culbm_ctx* ctx) {
 }
 
 // Workgroup order of a launch over planes [z0, z1) (see cell_block), measured
-// per form at 128^3 to 384^3.  Dispatch order is the faster one for pull-form
-// launches, whole-domain or wavefront chunk (288x272x280: FP32 pull +6.6 %, FP16
-// pull +11 %, FP16 pull wavefront of 16-plane chunks 46.8k vs 39.0k MLUPS), and
-// for whole-domain in-place steps.  The XCD-contiguous remap stays on the
-// in-place wavefront chunks (within 1 % either way there) and on whole-domain
-// gather launches (pull, adjoint) below 2^22 cells, where each XCD's L2 serving
-// its own slab's neighbour planes wins (FP32 pull at 128^3: +5 %).
+// per form at 128^3 to 384^3.  Dispatch order, which hands consecutive
+// workgroups to the 8 XCDs in turn, is the faster one for every wavefront chunk
+// (288x272x280 at the chosen depths: FP32 in place 25.7k vs 21.3k MLUPS, FP16
+// pull 46.8k vs 39.0k) and for whole-domain steps (FP32 pull +6.6 %, FP16 pull
+// +11 %).  The XCD-contiguous remap stays only on whole-domain gather launches
+// (pull, adjoint) below 2^22 cells, where each XCD's L2 serving its own slab's
+// neighbour planes wins (FP32 pull at 128^3: +5 %).
 constexpr uint32_t kRemapGatherCells = 1u << 22;
 
 void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather) {
@@ -250,8 +250,8 @@ void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather) {
   a.z1 = z1;
   a.ncells = uint32_t(z1 - z0) * uint32_t(a.nx) * uint32_t(a.ny);
   a.nblk = (a.ncells + kBlock - 1) / kBlock;
-  const bool part = z0 > 0 || z1 < a.nz;
-  a.xcd = (part ? !gather : (gather && a.ncells < kRemapGatherCells)) ? 1u : 0u;
+  const bool whole = z0 == 0 && z1 == a.nz;
+  a.xcd = (whole && gather && a.ncells < kRemapGatherCells) ? 1u : 0u;
 }
 
 // Event pair around one launch when profiling is on.
@@ -556,13 +556,13 @@ int alloc(culbm_ctx* ctx, void** p, size_t bytes) {
 //  * fit: the working set (D + T + 1 planes of every buffer plus flags) within
 //    ~160 MB of the 256 MB Infinity Cache;
 //  * floor: enough cells per launch that ramp-up and tail stay small.
-// Measured on 288x272 planes (innerLoop 5): FP32 in-place runs best at 32 planes
-// (2.5M cells per launch, 8 % above the cache-fitting 13); 16-bit in-place at 16
-// planes (1.25M cells); at 128^3 the cache-fitting 83 planes beat a single
-// chunk of the whole domain by 16 %, so the floor never takes the depth past
-// half the domain.  The pull wavefront (chunks in dispatch order) runs best at
-// about 1.25M cells per chunk in both widths from 128^3 to 384^3 (FP16 +4 to
-// +8 % over 2.5M, FP32 -1 to +4.5 %).
+// Measured with the chunks in dispatch order (innerLoop 5, 256^3, 288x272x280,
+// 384^3): FP32 in place and pull in both widths run best at about 1.25M cells
+// per chunk (FP32 in place at 288x272x280: 25.7k MLUPS at 16 planes, 22.7k at
+// 32; FP16 pull 46.7k at 16 planes, 44.5k at 32), 16-bit in place at about
+// 2.5M (45.0k at 32 planes, 41.6k at 16).  At 128^3 the cache-fitting 83
+// planes beat a single chunk of the whole domain by 16 %, so the floor never
+// takes the depth past half the domain.
 int32_t auto_depth(const culbm_ctx* ctx) {
   const double cells = double(ctx->cfg.nx) * ctx->cfg.ny;
   const double plane = cells * (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
@@ -570,10 +570,10 @@ int32_t auto_depth(const culbm_ctx* ctx) {
   const int32_t T = std::max(1, ctx->cfg.inner_loop);
   const int32_t nz = ctx->cfg.nz;
   const int32_t fit = std::max(int32_t(budget / plane) - T - 1, 2);
-  const bool narrow = !ctx->inplace || ctx->bpp == 2;
-  const int32_t floor_ = int32_t(std::ceil((narrow ? 1.25e6 : 2.5e6) / cells));
-  const int32_t d = narrow ? std::min(floor_, std::max(fit, nz / 2))
-                           : std::min(std::max(fit, floor_), std::max(fit, nz / 2));
+  const bool wide = ctx->inplace && ctx->bpp == 2;
+  const int32_t floor_ = int32_t(std::ceil((wide ? 2.5e6 : 1.25e6) / cells));
+  const int32_t d = wide ? std::min(std::max(fit, floor_), std::max(fit, nz / 2))
+                         : std::min(floor_, std::max(fit, nz / 2));
   return std::max(2, std::min(d, nz));
 }
 

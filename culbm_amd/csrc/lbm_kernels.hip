@@ -35,9 +35,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblk) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// The workgroup order of a cell launch.  Launches over the whole domain run
-// in dispatch order: there the remap cost 6 % (FP32 pull) to 11 % (FP16 pull),
-// while the plane-range launches of the wavefront gain from it (2 % in FP16).
+// The workgroup order of a cell launch: dispatch order, or the XCD-contiguous
+// remap when the host sets a.xcd (whole-domain gather launches on domains below
+// 2^22 cells, where each XCD's L2 then serves its own slab's neighbour planes).
 __device__ __forceinline__ uint32_t cell_block(const StepArgs& a) {
   return a.xcd ? xcd_block(blockIdx.x, a.nblk) : blockIdx.x;
 }

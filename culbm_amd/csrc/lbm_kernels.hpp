@@ -121,7 +121,7 @@ struct StepArgs {
   int64_t dstride;       // elements between consecutive directions
   FastDiv divx, divy;    // divide by nx, by ny
   uint32_t nblk;         // workgroups in the launch (for the XCD remap)
-  uint32_t xcd;          // 1: remap workgroups to XCD-contiguous slabs (plane-range launches)
+  uint32_t xcd;  // 1: remap workgroups to XCD-contiguous slabs (cell_block)
   const uint8_t* sflag;  // per-cell step bits (kStepBits of the flag word)
   const float* rho;
   const float* vx;
