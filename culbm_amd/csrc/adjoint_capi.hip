@@ -225,7 +225,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   a.px = c.nx;
   a.py = c.ny;
   a.dstride = n;
-  set_range(a, 0, c.nz, true);
+  set_range(a, 0, c.nz, true, ctx->bpp == 2);
 
   // host-side index lists
   std::vector<uint8_t> fl(static_cast<size_t>(n));

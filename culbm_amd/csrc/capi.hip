@@ -236,22 +236,26 @@ StepArgs base_args(const culbm_ctx* ctx) {
 }
 
 // Workgroup order of a launch over planes [z0, z1) (see cell_block), measured
-// per form at 128^3 to 384^3.  Dispatch order, which hands consecutive
-// workgroups to the 8 XCDs in turn, is the faster one for every wavefront chunk
-// (288x272x280 at the chosen depths: FP32 in place 25.7k vs 21.3k MLUPS, FP16
-// pull 46.8k vs 39.0k) and for whole-domain steps (FP32 pull +6.6 %, FP16 pull
-// +11 %).  The XCD-contiguous remap stays only on whole-domain gather launches
+// per form from 128^3 to 384^3.  Runs of 8 workgroups per XCD, dealt round
+// robin, keep the 8 XCDs on one streaming front; they beat XCD-contiguous slabs
+// for every wavefront chunk (288x272x280 at the chosen depths: FP32 in place
+// 26.1k vs 21.3k MLUPS) and whole-domain step, and plain dispatch order in
+// place (+0.2 to +8 %) and for FP32 pull (+0.5 to +5.6 %).  16-bit pull stays in
+// dispatch order: there runs gain 1-2 % on most sizes but lose 13 % on the
+// 128^3 wavefront and 4 % at 256^3.  Slabs stay on whole-domain gather launches
 // (pull, adjoint) below 2^22 cells, where each XCD's L2 serving its own slab's
-// neighbour planes wins (FP32 pull at 128^3: +5 %).
+// neighbour planes wins (FP32 pull at 128^3: +5 % over dispatch order).
 constexpr uint32_t kRemapGatherCells = 1u << 22;
 
-void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather) {
+void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather, bool narrow) {
   a.z0 = z0;
   a.z1 = z1;
   a.ncells = uint32_t(z1 - z0) * uint32_t(a.nx) * uint32_t(a.ny);
   a.nblk = (a.ncells + kBlock - 1) / kBlock;
   const bool whole = z0 == 0 && z1 == a.nz;
-  a.xcd = (whole && gather && a.ncells < kRemapGatherCells) ? 1u : 0u;
+  a.xcd = (whole && gather && a.ncells < kRemapGatherCells) ? kOrderSlabs
+          : (gather && narrow)                              ? kOrderDispatch
+                                                            : kOrderRuns;
 }
 
 // Event pair around one launch when profiling is on.
@@ -288,7 +292,7 @@ void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity) {
 
 template <class S>
 int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) {
-  set_range(a, z0, z1, !ctx->inplace);
+  set_range(a, z0, z1, !ctx->inplace, ctx->bpp == 2);
   if (a.ncells == 0) return CULBM_OK;
   if (int rc = prof_begin(ctx)) return rc;
   if (ctx->inplace) {
@@ -461,7 +465,7 @@ int steps_dispatch(culbm_ctx* ctx, int32_t nsteps) {
 
 StepArgs full_args(const culbm_ctx* ctx) {
   StepArgs a = base_args(ctx);
-  set_range(a, 0, ctx->cfg.nz, false);
+  set_range(a, 0, ctx->cfg.nz, false, false);
   return a;
 }
 
@@ -557,12 +561,13 @@ int alloc(culbm_ctx* ctx, void** p, size_t bytes) {
 //    ~160 MB of the 256 MB Infinity Cache;
 //  * floor: enough cells per launch that ramp-up and tail stay small.
 // Measured with the chunks in dispatch order (innerLoop 5, 256^3, 288x272x280,
-// 384^3): FP32 in place and pull in both widths run best at about 1.25M cells
-// per chunk (FP32 in place at 288x272x280: 25.7k MLUPS at 16 planes, 22.7k at
-// 32; FP16 pull 46.7k at 16 planes, 44.5k at 32), 16-bit in place at about
-// 2.5M (45.0k at 32 planes, 41.6k at 16).  At 128^3 the cache-fitting 83
-// planes beat a single chunk of the whole domain by 16 %, so the floor never
-// takes the depth past half the domain.
+// 384^3): FP32 in place runs best at about 1.4M cells per chunk (288x272x280:
+// 26.1k MLUPS at 18 planes, 25.8k at 16, 24.0k at 22, 22.7k at 32), 16-bit in
+// place at about 2.6M (45.3k at 34 planes, 41.6k at 16, 41.2k at 40) and pull in
+// both widths at about 1.25M (FP16 46.7k at 16 planes, 44.5k at 32).  The rate
+// falls off once the chunk's working set nears the 256 MB cache.  At 128^3 the
+// cache-fitting 83 planes beat a single chunk of the whole domain by 16 %, so
+// the floor never takes the depth past half the domain.
 int32_t auto_depth(const culbm_ctx* ctx) {
   const double cells = double(ctx->cfg.nx) * ctx->cfg.ny;
   const double plane = cells * (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
@@ -571,7 +576,8 @@ int32_t auto_depth(const culbm_ctx* ctx) {
   const int32_t nz = ctx->cfg.nz;
   const int32_t fit = std::max(int32_t(budget / plane) - T - 1, 2);
   const bool wide = ctx->inplace && ctx->bpp == 2;
-  const int32_t floor_ = int32_t(std::ceil((wide ? 2.5e6 : 1.25e6) / cells));
+  const double target = !ctx->inplace ? 1.25e6 : wide ? 2.6e6 : 1.4e6;
+  const int32_t floor_ = int32_t(std::ceil(target / cells));
   const int32_t d = wide ? std::min(std::max(fit, floor_), std::max(fit, nz / 2))
                          : std::min(floor_, std::max(fit, nz / 2));
   return std::max(2, std::min(d, nz));

@@ -35,11 +35,29 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblk) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// The workgroup order of a cell launch: dispatch order, or the XCD-contiguous
-// remap when the host sets a.xcd (whole-domain gather launches on domains below
-// 2^22 cells, where each XCD's L2 then serves its own slab's neighbour planes).
+// Runs of kXcdGroup consecutive workgroups per XCD, the runs dealt round robin:
+// the 8 XCDs still advance through the cells side by side (one streaming front,
+// as in dispatch order), while the lines two neighbouring workgroups share (the
+// one-element x shift of 9 directions at every 256-cell edge) mostly stay in one
+// L2.  Whole super-groups of 8 * kXcdGroup workgroups are permuted; a partial
+// last one keeps dispatch order.  At 288x272x280 against dispatch order: FP32
+// in-place wavefront +1.4 %, FP16 +4 %, FP32 pull +2.3 % (runs of 2 to 32
+// measured; 8 and 16 lead in every form).
+constexpr uint32_t kXcdGroup = 8;
+
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t nblk) {
+  constexpr uint32_t G = 8u * kXcdGroup;
+  const uint32_t sg = b / G;
+  if ((sg + 1) * G > nblk) return b;
+  const uint32_t r = b - sg * G;
+  return sg * G + (r & 7u) * kXcdGroup + (r >> 3);
+}
+
+// The workgroup order of a cell launch, chosen by the host per launch (a.xcd):
+// kOrderDispatch, kOrderSlabs (xcd_block) or kOrderRuns (xcd_group).
 __device__ __forceinline__ uint32_t cell_block(const StepArgs& a) {
-  return a.xcd ? xcd_block(blockIdx.x, a.nblk) : blockIdx.x;
+  const uint32_t b = blockIdx.x;
+  return a.xcd == kOrderSlabs ? xcd_block(b, a.nblk) : a.xcd == kOrderRuns ? xcd_group(b, a.nblk) : b;
 }
 
 struct Cell {
