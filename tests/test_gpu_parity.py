@@ -249,21 +249,39 @@ def test_set_get_ddf_roundtrip(sp, op, storage):
 
 # ---- the optimised workload at full size --------------------------------------
 
+FULL_SIZE_FORMS = {
+    "fp32": [(1, 3), (0, 3), (0, 2), (1, 1), (1, 0)],
+    "fp16": [(1, 3), (0, 3), (0, 2)],
+    "bf16": [(1, 3)],
+}
+
+
 @pytest.mark.slow
-def test_full_size_blocked_equals_plain():
-    """288x272x280, 60 spheres: in-place + wavefront blocking (I = 5) vs plain pull."""
+@pytest.mark.parametrize("storage,rounding", [("fp32", "nearest"), ("fp16", "nearest"), ("fp16", "stochastic"),
+                                              ("bf16", "nearest")])
+def test_full_size_blocked_equals_plain(storage, rounding):
+    """288x272x280, 60 spheres, I = 5, automatic depths and workgroup orders (the
+    benchmark's configurations): every blocked and in-place form vs plain pull."""
     st = cases.config_obstacles()
     steps = 25
-    out = []
-    for sp, op in ((0, 0), (1, 3)):
-        lat = make(st, inv_tau=0.5, stream_policy=sp, opt_policy=op, inner_loop=5)
+
+    def run(sp, op):
+        lat = make(st, inv_tau=0.5, stream_policy=sp, opt_policy=op, inner_loop=5, storage=storage,
+                   rounding=rounding)
         lat.step(steps)
         lat.dump()
-        out.append((lat.macros(), lat.get_ddf()))
+        r = (lat.macros(), lat.get_ddf())
         lat.close()
-    for a, b in zip(out[0][0], out[1][0]):
-        assert_bitexact(a, b, "full-size macros")
-    assert_bitexact(out[0][1], out[1][1], "full-size populations")
+        return r
+
+    want_m, want_f = run(0, 0)
+    for sp, op in FULL_SIZE_FORMS[storage]:
+        got_m, got_f = run(sp, op)
+        what = f"full-size ({sp},{op}) {storage} {rounding}"
+        for a, b in zip(got_m, want_m):
+            assert_bitexact(a, b, what + " macros")
+        assert_bitexact(got_f, want_f, what + " populations")
+        del got_m, got_f
 
 
 def test_precision_workload_512_blocked_equals_plain():
