@@ -23,3 +23,15 @@ def test_population_pattern_is_faster_from_the_cache_than_from_hbm():
     small = probe_bandwidth("pop27", 64 * MB, iters=5)
     big = probe_bandwidth("pop27", 1024 * MB, iters=3)
     assert small > 0.9 * big, (small, big)
+
+
+def test_selector_times_every_candidate_and_returns_the_fastest():
+    from culbm_amd.calibrate import select
+
+    res = select((64, 48, 40), stream_policy=1, storage="fp32", inner_loops=(3,), depths=(0, 4, 8), steps=6)
+    rows = res["candidates"]
+    assert [r["requested_depth"] for r in rows] == [0, 4, 8]
+    assert all(r["mlups"] > 0 for r in rows)
+    assert [r["wave_depth"] for r in rows[1:]] == [4, 8]  # explicit depths are used as given
+    assert rows[0]["wave_depth"] >= 2                      # the automatic one is reported
+    assert res["best"]["mlups"] == max(r["mlups"] for r in rows)
