@@ -242,7 +242,8 @@ template <class S>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
     const AdjArgs j) {
   const StepArgs& a = j.a;
-  const Cell k = locate(a, cell_block(a));
+  const uint32_t blk = cell_block(a);
+  const Cell k = locate(a, blk);
   float dtau = 0.0f;
   if (k.ok) {
     float h[27], lam[27];
@@ -301,8 +302,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
     for (int i = 0; i < 27; ++i) st_at<float>(out + i * (a.dstride * 4), k.l * 4u, lam[i]);
   }
+  // one accumulator per block of cells (not per hardware workgroup), so the
+  // fixed-order host sum does not depend on the workgroup order
   const double s = block_sum(double(dtau));
-  if (threadIdx.x == 0) j.tau_acc[blockIdx.x] += s;
+  if (threadIdx.x == 0) j.tau_acc[blk] += s;
 }
 
 // lambda^0 = dJ/df^0 (plain FP32 [27][n]): consumers' mu^0, plus the carry of
