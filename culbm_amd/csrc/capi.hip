@@ -253,7 +253,7 @@ void set_range(StepArgs& a, int32_t z0, int32_t z1, bool gather, bool narrow) {
   a.ncells = uint32_t(z1 - z0) * uint32_t(a.nx) * uint32_t(a.ny);
   a.nblk = (a.ncells + kBlock - 1) / kBlock;
   const bool whole = z0 == 0 && z1 == a.nz;
-  a.xcd = (whole && gather && a.ncells < kRemapGatherCells) ? kOrderSlabs
+  a.order = (whole && gather && a.ncells < kRemapGatherCells) ? kOrderSlabs
           : (gather && narrow)                              ? kOrderDispatch
                                                             : kOrderRuns;
 }

@@ -53,11 +53,11 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t nblk) {
   return sg * G + (r & 7u) * kXcdGroup + (r >> 3);
 }
 
-// The workgroup order of a cell launch, chosen by the host per launch (a.xcd):
+// The workgroup order of a cell launch, chosen by the host per launch (a.order):
 // kOrderDispatch, kOrderSlabs (xcd_block) or kOrderRuns (xcd_group).
 __device__ __forceinline__ uint32_t cell_block(const StepArgs& a) {
   const uint32_t b = blockIdx.x;
-  return a.xcd == kOrderSlabs ? xcd_block(b, a.nblk) : a.xcd == kOrderRuns ? xcd_group(b, a.nblk) : b;
+  return a.order == kOrderSlabs ? xcd_block(b, a.nblk) : a.order == kOrderRuns ? xcd_group(b, a.nblk) : b;
 }
 
 struct Cell {

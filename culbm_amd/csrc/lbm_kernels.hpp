@@ -107,17 +107,17 @@ struct StoreBF16 {
   }
 };
 
+// Workgroup orders of a cell launch (StepArgs::order).
+constexpr uint32_t kOrderDispatch = 0;  // blockIdx as dispatched, round robin over the XCDs
+constexpr uint32_t kOrderSlabs = 1;     // one contiguous slab of workgroups per XCD
+constexpr uint32_t kOrderRuns = 2;      // runs of consecutive workgroups per XCD, dealt round robin
+
 // Arguments of one step launch over the z-plane range [z0, z1).
 //
 // Domain arrays (flags, rho/u) are dense [nz][ny][nx].  Population storage is
 // [27][pz][py][px] with direction stride `dstride`; the pull layout uses
 // px=nx, py=ny, pz=nz, the in-place layout pads every axis by one cell so the
 // twisted store at x+1 of the last cell has a home.
-// Workgroup orders of a cell launch (StepArgs::xcd).
-constexpr uint32_t kOrderDispatch = 0;  // blockIdx as dispatched, round robin over the XCDs
-constexpr uint32_t kOrderSlabs = 1;     // one contiguous slab of workgroups per XCD
-constexpr uint32_t kOrderRuns = 2;      // runs of consecutive workgroups per XCD, dealt round robin
-
 struct StepArgs {
   int32_t nx, ny, nz;
   int32_t px, py;        // layout pitches (cells per row, rows per plane)
@@ -126,7 +126,7 @@ struct StepArgs {
   int64_t dstride;       // elements between consecutive directions
   FastDiv divx, divy;    // divide by nx, by ny
   uint32_t nblk;         // workgroups in the launch (for the XCD remap)
-  uint32_t xcd;  // workgroup order: kOrderDispatch / kOrderSlabs / kOrderRuns (cell_block)
+  uint32_t order;         // workgroup order: kOrderDispatch / kOrderSlabs / kOrderRuns (cell_block)
   const uint8_t* sflag;  // per-cell step bits (kStepBits of the flag word)
   const float* rho;
   const float* vx;
