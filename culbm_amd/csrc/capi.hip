@@ -556,30 +556,35 @@ int alloc(culbm_ctx* ctx, void** p, size_t bytes) {
   return CULBM_OK;
 }
 
-// Planes per wavefront chunk, from two limits:
-//  * fit: the working set (D + T + 1 planes of every buffer plus flags) within
-//    ~160 MB of the 256 MB Infinity Cache;
-//  * floor: enough cells per launch that ramp-up and tail stay small.
-// Measured with the chunks in dispatch order (innerLoop 5, 256^3, 288x272x280,
-// 384^3): FP32 in place runs best at about 1.4M cells per chunk (288x272x280:
-// 26.1k MLUPS at 18 planes, 25.8k at 16, 24.0k at 22, 22.7k at 32), 16-bit in
-// place at about 2.6M (45.3k at 34 planes, 41.6k at 16, 41.2k at 40) and pull in
-// both widths at about 1.25M (FP16 46.7k at 16 planes, 44.5k at 32).  The rate
-// falls off once the chunk's working set nears the 256 MB cache.  At 128^3 the
-// cache-fitting 83 planes beat a single chunk of the whole domain by 16 %, so
-// the floor never takes the depth past half the domain.
+// Planes per wavefront chunk.  FP32 in place and pull: a launch-size target
+// (1.4M cells in place, 1.25M pull) capped by the cache-fitting depth (D + T + 1
+// planes within ~160 MB of the 256 MB Infinity Cache) or half the domain,
+// whichever is larger.  16-bit in place: the deepest chunk whose working set,
+// D + T + 1 planes of the padded in-place layout, stays within 190 MB, but at
+// least 1.6M cells per launch.  Measured with the chunks in runs of 8 workgroups
+// per XCD, innerLoop 5: FP32 in place at 288x272x280 runs 26.1k MLUPS at 18
+// planes, 24.0k at 22, 22.7k at 32 (the rate falls off once the working set
+// nears the cache; at 256^3 and 384^3 the optimum moved by a plane or two from
+// run to run, within +-4 %); FP16 pull 46.7k at 16 planes, 44.5k at 32.  FP16
+// in place peaked at 172-190 MB in every run: 256^3 46.2k at 33 planes, 42.2k
+// at 40; 384^3 47.7k at 12, 44.7k at 18; 288x272x280 44.9-48.2k at 32-34.  At
+// 128^3 the cache-fitting 83 planes beat one whole-domain chunk by 16 % (FP32).
 int32_t auto_depth(const culbm_ctx* ctx) {
   const double cells = double(ctx->cfg.nx) * ctx->cfg.ny;
-  const double plane = cells * (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
-  const double budget = 160.0 * 1024 * 1024;
   const int32_t T = std::max(1, ctx->cfg.inner_loop);
   const int32_t nz = ctx->cfg.nz;
+  if (ctx->inplace && ctx->bpp == 2) {
+    const double plane = 27.0 * double(ctx->px) * double(ctx->py) * ctx->bpp + cells;
+    const int32_t fit = int32_t(190e6 / plane) - T - 1;
+    const int32_t d = std::max(fit, int32_t(std::ceil(1.6e6 / cells)));
+    return std::max(2, std::min(d, nz));
+  }
+  const double plane = cells * (27.0 * ctx->bpp * (ctx->inplace ? 1 : 2) + 1.0);
+  const double budget = 160.0 * 1024 * 1024;
   const int32_t fit = std::max(int32_t(budget / plane) - T - 1, 2);
-  const bool wide = ctx->inplace && ctx->bpp == 2;
-  const double target = !ctx->inplace ? 1.25e6 : wide ? 2.6e6 : 1.4e6;
+  const double target = ctx->inplace ? 1.4e6 : 1.25e6;
   const int32_t floor_ = int32_t(std::ceil(target / cells));
-  const int32_t d = wide ? std::min(std::max(fit, floor_), std::max(fit, nz / 2))
-                         : std::min(floor_, std::max(fit, nz / 2));
+  const int32_t d = std::min(floor_, std::max(fit, nz / 2));
   return std::max(2, std::min(d, nz));
 }
 
