@@ -86,6 +86,8 @@ def select(domain: Tuple[int, int, int], stream_policy: int = 1, storage: str = 
         state = cases.config_obstacles(nx, ny, nz) if min(domain) >= 64 else cases.channel(nx, ny, nz)
     cands = list(depths) if depths is not None else depth_candidates(domain)
     inner_loops = list(inner_loops)
+    if not cands or not inner_loops:
+        raise ValueError("select needs at least one depth and one innerLoop value")
 
     def timed(il, d):
         lat = Lattice(LatticeConfig(domain=domain, stream_policy=stream_policy, opt_policy=3, inner_loop=il,

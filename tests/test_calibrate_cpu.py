@@ -21,3 +21,14 @@ def test_depth_candidates_at_the_optimised_workload():
     # 288x272 planes of 78,336 cells: 1.4M cells is 18 planes, 2.6M is 34
     c = depth_candidates((288, 272, 280))
     assert 18 in c and 34 in c and 16 in c
+
+
+def test_select_rejects_an_empty_candidate_list():
+    import pytest
+
+    from culbm_amd.calibrate import select
+
+    with pytest.raises(ValueError):
+        select((16, 16, 16), depths=())
+    with pytest.raises(ValueError):
+        select((16, 16, 16), inner_loops=())
