@@ -123,6 +123,7 @@ SIGNATURES = {
     "culbm_probe_cells": (ctypes.c_int, [_I32, _I32, ctypes.c_float, _VP, _VP, _VP, ctypes.c_int64]),
     "culbm_probe_codec": (ctypes.c_int, [_I32, _I32, ctypes.c_float, _F32, _VP, _F32, ctypes.c_int64]),
     "culbm_blocking_axis": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_I32), _VP, _I32]),
+    "culbm_wave_schedule": (ctypes.c_int, [_I32, _I32, _I32, _VP, _I32, ctypes.POINTER(_I32)]),
     "culbm_probe_bandwidth": (ctypes.c_int, [_I32, _I32, ctypes.c_int64, _I32, ctypes.POINTER(ctypes.c_float)]),
 }
 

@@ -83,3 +83,15 @@ def redundancy(domain: Tuple[int, int, int], tile: Tuple[int, int, int], it: int
     total = sum(int(np.prod([e - o for o, e in zip(t.origin, t.load_end)]))
                 for t in plan(domain, tile, it))
     return total / float(np.prod(domain))
+
+
+def wave_schedule(nz: int, depth: int, steps: int) -> List[Tuple[int, int, int]]:
+    """Launches (z0, z1, t) of one pass of the dynamic / static policies in issue
+    order, as the library generates them (culbm_wave_schedule): step t
+    (1-based) over planes [z0, z1)."""
+    cnt = ctypes.c_int32()
+    _lib.check(_lib.lib().culbm_wave_schedule(int(nz), int(depth), int(steps), None, 0, ctypes.byref(cnt)))
+    out = np.zeros(3 * cnt.value, np.int32)
+    _lib.check(_lib.lib().culbm_wave_schedule(int(nz), int(depth), int(steps), _lib.ptr(out), cnt.value,
+                                              ctypes.byref(cnt)))
+    return [tuple(int(v) for v in out[3 * k: 3 * k + 3]) for k in range(cnt.value)]

@@ -92,6 +92,44 @@ bool blocked(const culbm_config& c) { return c.opt_policy != CULBM_OPT_NONE; }
 // range; BF16 has FP32's exponent range and needs none.
 float default_scale(int32_t storage) { return storage == CULBM_STORAGE_FP16 ? 4096.0f : 1.0f; }
 
+// Population layout of a configuration: pitches and direction stride.  The one
+// place that defines it, so the 32-bit offset check of culbm_validate tests
+// exactly what culbm_create allocates.
+struct Layout {
+  int32_t px, py, pz;
+  int64_t dstride;  // elements per direction array
+};
+
+Layout layout_of(const culbm_config& c) {
+  Layout L{};
+  if (c.stream_policy == CULBM_STREAM_INPLACE) {
+    // Row pitch nx + 1 rounded up to 16 elements (64 B in FP32, 32 B in 16-bit),
+    // not to a 128-byte line: with whole-line rows the ghost column of a row
+    // whose nx + 1 just passes a line boundary (nx = 288: 289 of 320 elements)
+    // sits alone in a line written 4 bytes at a time.  Measured against the
+    // 128-byte pitch: FP32 in place + wavefront +2.7 % at 288x272x280, +6.6 %
+    // at 289x272x280, +8.7 % at 384^3 (-3 % at 512^3); FP16 +1 to +7 %; FP32
+    // in place without blocking +7 to +16 %.  (An unpadded pitch, nx + 1,
+    // helped the unblocked step most but cost 10 % at 289x272x280 blocked.)
+    // Without blocking, 8 elements measured faster still: FP32 +3 to +13 %
+    // (21.1k vs 18.8k MLUPS at 288x272x280), 16-bit +1.5 to +3 %.
+    const int64_t align = c.opt_policy == CULBM_OPT_NONE ? 8 : 16;
+    L.px = int32_t(round_up(int64_t(c.nx) + 1, align));
+    L.py = c.ny + 1;
+    L.pz = c.nz + 1;
+  } else {
+    L.px = c.nx;
+    L.py = c.ny;
+    L.pz = c.nz;
+  }
+  L.dstride = round_up(int64_t(L.px) * L.py * L.pz, 256);
+  return L;
+}
+
+int32_t bytes_per_pop(int32_t storage) {
+  return (storage == CULBM_STORAGE_FP16 || storage == CULBM_STORAGE_BF16) ? 2 : 4;
+}
+
 }  // namespace
 
 extern "C" {
@@ -132,14 +170,15 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
     return bad(CULBM_ERR_INVALID, "domain dimensions must be >= 1 (got %d x %d x %d)", c->nx, c->ny,
                c->nz);
   const int64_t n = int64_t(c->nx) * c->ny * c->nz;
-  const int64_t padded = round_up(int64_t(c->nx) + 1, 64) * (int64_t(c->ny) + 1) * (int64_t(c->nz) + 1);
-  if (n > (int64_t(1) << 31) - 1 || padded > (int64_t(1) << 32) - 1)
+  if (n > (int64_t(1) << 31) - 1)
     return bad(CULBM_ERR_INVALID, "domain of %lld cells exceeds the 32-bit cell index", (long long)n);
   {
     // the kernels address one direction array with 32-bit per-lane byte offsets
-    const int64_t bpp = (c->storage == CULBM_STORAGE_FP16 || c->storage == CULBM_STORAGE_BF16) ? 2 : 4;
-    const int64_t per_dir = (c->stream_policy == CULBM_STREAM_INPLACE ? padded : n) * bpp;
-    if (per_dir > (int64_t(1) << 32))  // the last element starts at per_dir - bpp
+    // (cell index x + px * (y + py * z) of the layout culbm_create allocates)
+    const Layout L = layout_of(*c);
+    const int64_t cells = int64_t(L.px) * L.py * L.pz;
+    const int64_t per_dir = cells * bytes_per_pop(c->storage);
+    if (cells > (int64_t(1) << 32) - 1 || per_dir > (int64_t(1) << 32))  // the last element starts at per_dir - bpp
       return bad(CULBM_ERR_INVALID,
                  "domain too large: one direction array takes %lld bytes, above the 4 GiB the "
                  "kernels' 32-bit offsets reach (use 16-bit storage or a smaller domain)",
@@ -311,31 +350,40 @@ int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) 
   return prof_end(ctx);
 }
 
-// One temporal-blocking pass of T steps as a skewed z-wavefront.  Chunk k at
-// step t (1-based) covers planes [kD - (t-1), (k+1)D - (t-1)): every plane it
-// reads at level t-1 was produced either by an earlier chunk or by the same
-// chunk one launch earlier, and no (plane, step) is computed twice.  A chunk's
-// working set (D + T planes) is sized for the Infinity Cache.
-template <class S>
-int wave_pass(culbm_ctx* ctx, int32_t T) {
-  StepArgs a = base_args(ctx);
-  const int32_t nz = ctx->cfg.nz, D = ctx->wave_depth;
-  const uint32_t base = uint32_t(ctx->step);
+// The launches of one temporal-blocking pass of T steps as a skewed
+// z-wavefront, in issue order: f(lo, hi, t) for step t (1-based) over planes
+// [lo, hi).  Chunk k at step t covers planes [kD - (t-1), (k+1)D - (t-1)):
+// every plane it reads at level t-1 was produced either by an earlier chunk or
+// by the same chunk one launch earlier, and no (plane, step) is computed twice.
+// A chunk's working set (D + T planes) is sized for the Infinity Cache.  The
+// schedule is exported (culbm_wave_schedule) for the host-side hazard check.
+template <class F>
+int for_each_wave_launch(int32_t nz, int32_t D, int32_t T, F&& f) {
   const int32_t K = (nz + T - 1 + D - 1) / D;
   for (int32_t k = 0; k < K; ++k) {
     for (int32_t t = 1; t <= T; ++t) {
       const int32_t lo = std::max(0, k * D - (t - 1));
       const int32_t hi = std::min(nz, (k + 1) * D - (t - 1));
       if (lo >= hi) continue;
-      if (!ctx->inplace) {
-        a.src = ctx->buf[(ctx->cur + t - 1) & 1];
-        a.dst = ctx->buf[(ctx->cur + t) & 1];
-      }
-      a.state = base + uint32_t(t);
-      const int rc = launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1);
-      if (rc) return rc;
+      if (int rc = f(lo, hi, t)) return rc;
     }
   }
+  return CULBM_OK;
+}
+
+template <class S>
+int wave_pass(culbm_ctx* ctx, int32_t T) {
+  StepArgs a = base_args(ctx);
+  const uint32_t base = uint32_t(ctx->step);
+  const int rc = for_each_wave_launch(ctx->cfg.nz, ctx->wave_depth, T, [&](int32_t lo, int32_t hi, int32_t t) {
+    if (!ctx->inplace) {
+      a.src = ctx->buf[(ctx->cur + t - 1) & 1];
+      a.dst = ctx->buf[(ctx->cur + t) & 1];
+    }
+    a.state = base + uint32_t(t);
+    return launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1);
+  });
+  if (rc) return rc;
   ctx->cur = (ctx->cur + T) & 1;
   ctx->parity = (ctx->parity + T) & 1;
   ctx->step += T;
@@ -643,31 +691,17 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
   ctx->n = int64_t(c.nx) * c.ny * c.nz;
   ctx->inplace = c.stream_policy == CULBM_STREAM_INPLACE;
   ctx->sr = c.rounding == CULBM_ROUND_STOCHASTIC;
-  ctx->bpp = c.storage == CULBM_STORAGE_FP32 ? 4 : 2;
+  ctx->bpp = bytes_per_pop(c.storage);
   ctx->scale = c.storage == CULBM_STORAGE_FP32 ? 1.0f
                : (c.storage_scale != 0.0f ? c.storage_scale : default_scale(c.storage));
   ctx->inv_scale = 1.0f / ctx->scale;
-  if (ctx->inplace) {
-    // Row pitch nx + 1 rounded up to 16 elements (64 B in FP32, 32 B in 16-bit),
-    // not to a 128-byte line: with whole-line rows the ghost column of a row
-    // whose nx + 1 just passes a line boundary (nx = 288: 289 of 320 elements)
-    // sits alone in a line written 4 bytes at a time.  Measured against the
-    // 128-byte pitch: FP32 in place + wavefront +2.7 % at 288x272x280, +6.6 %
-    // at 289x272x280, +8.7 % at 384^3 (-3 % at 512^3); FP16 +1 to +7 %; FP32
-    // in place without blocking +7 to +16 %.  (An unpadded pitch, nx + 1,
-    // helped the unblocked step most but cost 10 % at 289x272x280 blocked.)
-    // Without blocking, 8 elements measured faster still: FP32 +3 to +13 %
-    // (21.1k vs 18.8k MLUPS at 288x272x280), 16-bit +1.5 to +3 %.
-    const int64_t align = c.opt_policy == CULBM_OPT_NONE ? 8 : 16;
-    ctx->px = int32_t(round_up(int64_t(c.nx) + 1, align));
-    ctx->py = c.ny + 1;
-    ctx->pz = c.nz + 1;
-  } else {
-    ctx->px = c.nx;
-    ctx->py = c.ny;
-    ctx->pz = c.nz;
+  {
+    const Layout L = layout_of(c);
+    ctx->px = L.px;
+    ctx->py = L.py;
+    ctx->pz = L.pz;
+    ctx->dstride = L.dstride;
   }
-  ctx->dstride = round_up(int64_t(ctx->px) * ctx->py * ctx->pz, 256);
 
   const size_t nb = size_t(ctx->n);
   CK(alloc(ctx, reinterpret_cast<void**>(&ctx->flag), nb * 4));
@@ -981,14 +1015,34 @@ int culbm_get_info(culbm_ctx* ctx, culbm_info* info) {
   if (!blocked(c)) {
     info->launches_per_pass = 1;
   } else {
-    const int32_t T = c.inner_loop, D = ctx->wave_depth, nz = c.nz;
-    const int32_t K = (nz + T - 1 + D - 1) / D;
     int32_t cnt = 0;
-    for (int32_t k = 0; k < K; ++k)
-      for (int32_t t = 1; t <= T; ++t)
-        if (std::max(0, k * D - (t - 1)) < std::min(nz, (k + 1) * D - (t - 1))) ++cnt;
+    for_each_wave_launch(c.nz, ctx->wave_depth, c.inner_loop, [&](int32_t, int32_t, int32_t) {
+      ++cnt;
+      return CULBM_OK;
+    });
     info->launches_per_pass = cnt;
   }
+  return CULBM_OK;
+}
+
+int culbm_wave_schedule(int32_t nz, int32_t depth, int32_t steps, int32_t* launches, int32_t max_launches,
+                        int32_t* count) {
+  if (!count) return fail(nullptr, CULBM_ERR_INVALID, "count is NULL");
+  if (nz < 1 || depth < 1 || steps < 1)
+    return fail(nullptr, CULBM_ERR_INVALID, "nz, depth and steps must be >= 1");
+  int32_t n = 0;
+  const int rc = for_each_wave_launch(nz, depth, steps, [&](int32_t lo, int32_t hi, int32_t t) {
+    if (launches) {
+      if (n >= max_launches) return fail(nullptr, CULBM_ERR_INVALID, "more than %d launches", max_launches);
+      launches[3 * n] = lo;
+      launches[3 * n + 1] = hi;
+      launches[3 * n + 2] = t;
+    }
+    ++n;
+    return int(CULBM_OK);
+  });
+  if (rc) return rc;
+  *count = n;
   return CULBM_OK;
 }
 
@@ -1075,52 +1129,89 @@ int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, in
 }
 
 int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs) {
-  if (bytes < 4096 || bytes > (int64_t(64) << 30) || iters < 1 || mode < 0 || mode > 6)
+  const bool wrong = (mode & CULBM_PROBE_CHECK_WRONG) != 0;
+  mode &= ~CULBM_PROBE_CHECK_WRONG;
+  // modes 3 and 4 launch one work-item per 4 bytes: 8 GiB keeps every mode's
+  // grid below 2^31 work-items
+  if (bytes < 4096 || bytes > (int64_t(8) << 30) || iters < 1 || mode < 0 || mode > 6)
     return fail(nullptr, CULBM_ERR_INVALID, "bad bandwidth probe arguments");
+  if (mode >= 5 && uint64_t(bytes) / 4 / 27 < 256)  // at least one 256-cell block per direction array
+    return fail(nullptr, CULBM_ERR_INVALID, "population-pattern probe needs at least %d bytes", 27 * 256 * 4);
   if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CULBM_ERR_HIP, "hipSetDevice failed");
   const uint64_t n4 = uint64_t(bytes) / 16;
+  const uint64_t nwords = 4 * n4;
   void *a = nullptr, *b = nullptr;
+  uint32_t* bad = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = CULBM_OK;
-  if (hipMalloc(&a, n4 * 16 + 256) != hipSuccess || hipMalloc(&b, n4 * 16 + 256) != hipSuccess)
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && !rc) rc = fail(nullptr, CULBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return rc == CULBM_OK;
+  };
+  if (hipMalloc(&a, n4 * 16 + 256) != hipSuccess || hipMalloc(&b, n4 * 16 + 256) != hipSuccess ||
+      hipMalloc(&bad, 256) != hipSuccess)
     rc = fail(nullptr, CULBM_ERR_HIP, "allocation failed");
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(nullptr, CULBM_ERR_HIP, "event creation failed");
+  // one element (or one cell) per thread
+  const uint32_t nb4 = uint32_t((n4 + kBlock - 1) / kBlock), nb1 = uint32_t((nwords + kBlock - 1) / kBlock);
+  // modes 5, 6: 27 arrays per buffer, each starting on a 1 KiB boundary like
+  // the population layout's direction stride
+  const uint64_t cells = nwords / 27 / 256 * 256;
+  const uint32_t nbc = uint32_t((cells + kBlock - 1) / kBlock);
+  const uint32_t seed_a = 0x1234567u, seed_b = 0x89abcdefu;
   if (!rc) {
-    // one element (or one cell) per thread
-    const uint32_t nb4 = uint32_t((n4 + kBlock - 1) / kBlock), nb1 = uint32_t((4 * n4 + kBlock - 1) / kBlock);
-    // modes 5, 6: 27 arrays per buffer, each starting on a 1 KiB boundary like
-    // the population layout's direction stride
-    const uint64_t cells = 4 * n4 / 27 / 256 * 256;
-    const uint32_t nbc = uint32_t((cells + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(a), n4);
-    hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4);
+    // known words in both buffers (64 words past the end included), so the
+    // check below sees exactly what the probe wrote
+    const uint64_t fw = nwords + 64;
+    const uint32_t nbf = uint32_t((fw + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bw_fill, dim3(nbf), dim3(kBlock), 0, 0, static_cast<uint32_t*>(a), fw, seed_a);
+    hipLaunchKernelGGL(k_bw_fill, dim3(nbf), dim3(kBlock), 0, 0, static_cast<uint32_t*>(b), fw, seed_b);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, 0, bad, uint64_t(64));
+    hip_ok(hipGetLastError(), "probe set-up launch");
+  }
+  if (!rc) {
     auto launch = [&]() {
       switch (mode) {
         case 0: hipLaunchKernelGGL(k_bw_copy, dim3(nb4), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float4*>(b), n4); break;
-        case 1: hipLaunchKernelGGL(k_bw_read, dim3(nb4), dim3(kBlock), 0, 0, static_cast<const float4*>(a), static_cast<float*>(b), n4); break;
+        case 1: hipLaunchKernelGGL(k_bw_read, dim3(nb4), dim3(kBlock), 0, 0, static_cast<const uint4*>(a), static_cast<uint32_t*>(b), n4); break;
         case 2: hipLaunchKernelGGL(k_bw_write, dim3(nb4), dim3(kBlock), 0, 0, static_cast<float4*>(b), n4); break;
-        case 3: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 0u); break;
-        case 4: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), n4 * 4, 1u); break;
+        case 3: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), nwords, 0u); break;
+        case 4: hipLaunchKernelGGL(k_bw_copy_shift, dim3(nb1), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), nwords, 1u); break;
         case 5: hipLaunchKernelGGL(k_bw_pop27, dim3(nbc), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), cells, 0u); break;
         default: hipLaunchKernelGGL(k_bw_pop27, dim3(nbc), dim3(kBlock), 0, 0, static_cast<const float*>(a), static_cast<float*>(b), cells, 1u); break;
       }
     };
     launch();  // warm-up
-    (void)hipEventRecord(e0, 0);
-    for (int32_t k = 0; k < iters; ++k) launch();
-    (void)hipEventRecord(e1, 0);
-    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(nullptr, CULBM_ERR_HIP, "probe kernel failed");
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    const double moved = (mode >= 5 ? double(cells) * 27.0 * 4.0 : double(n4) * 16.0) *
-                         (mode == 1 || mode == 2 ? 1.0 : 2.0) * iters;
-    if (gbs && ms > 0.0f) *gbs = float(moved / (double(ms) * 1e-3) / 1e9);
+    if (hip_ok(hipGetLastError(), "probe launch") && hip_ok(hipEventRecord(e0, 0), "hipEventRecord")) {
+      for (int32_t k = 0; k < iters; ++k) launch();
+      hip_ok(hipGetLastError(), "probe launch");
+    }
+    if (!rc && hip_ok(hipEventRecord(e1, 0), "hipEventRecord") &&
+        hip_ok(hipEventSynchronize(e1), "probe kernels")) {
+      float ms = 0.0f;
+      if (hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && !(ms > 0.0f))
+        rc = fail(nullptr, CULBM_ERR_HIP, "probe timing returned %g ms", double(ms));
+      const double moved = (mode >= 5 ? double(cells) * 27.0 * 4.0 : double(n4) * 16.0) *
+                           (mode == 1 || mode == 2 ? 1.0 : 2.0) * iters;
+      if (!rc && gbs) *gbs = float(moved / (double(ms) * 1e-3) / 1e9);
+    }
+  }
+  if (!rc) {
+    // the data check: every word the probe claims to have moved holds what it should
+    hipLaunchKernelGGL(k_bw_verify, dim3(4096), dim3(kBlock), 0, 0, int(mode), static_cast<const uint32_t*>(a),
+                       static_cast<const uint32_t*>(b), nwords, cells, uint64_t(nb4), seed_a, uint32_t(wrong), bad);
+    uint32_t nbad = 0;
+    if (hip_ok(hipGetLastError(), "probe check launch") &&
+        hip_ok(hipMemcpy(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost), "probe check copy") && nbad)
+      rc = fail(nullptr, CULBM_ERR_STATE, "bandwidth probe mode %d moved wrong data: %u words differ", int(mode),
+                nbad);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
+  if (bad) (void)hipFree(bad);
   return rc;
 }
 

@@ -905,15 +905,22 @@ __global__ __launch_bounds__(kBlock) void k_bw_copy(const float4* __restrict__ i
   if (i < n) out[i] = in[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_bw_read(const float4* __restrict__ in, float* out,
+// Read probe: each workgroup writes the XOR of the words it read (one word per
+// 4 KiB read, so the loads stay live and the check can recompute it).
+__global__ __launch_bounds__(kBlock) void k_bw_read(const uint4* __restrict__ in, uint32_t* out,
                                                     uint64_t n) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  float acc = 0.0f;
+  uint32_t acc = 0u;
   if (i < n) {
-    const float4 v = in[i];
-    acc = (v.x + v.y) + (v.z + v.w);
+    const uint4 v = in[i];
+    acc = (v.x ^ v.y) ^ (v.z ^ v.w);
   }
-  if (acc == 1.2345e-30f) out[0] = acc;  // keeps the loads alive
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc ^= __shfl_xor(acc, off, 64);
+  __shared__ uint32_t red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (red[0] ^ red[1]) ^ (red[2] ^ red[3]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bw_write(float4* __restrict__ out, uint64_t n) {
@@ -950,6 +957,61 @@ __global__ __launch_bounds__(kBlock) void k_bw_pop27(const float* __restrict__ i
   }
 #pragma unroll
   for (int i = 0; i < 27; ++i) out[uint64_t(i) * cells + c] = f[i];
+}
+
+// Data checks of the probes: the buffers start from known words, and after the
+// timed launches every word a probe claims to have moved must hold what it
+// should.  Word pattern of the source: a hash of the word index.
+__device__ __forceinline__ uint32_t probe_word(uint64_t i, uint32_t seed) {
+  return mix32(uint32_t(i) ^ mix32(uint32_t(i >> 32) + seed)) | 1u;  // never zero
+}
+
+__global__ __launch_bounds__(kBlock) void k_bw_fill(uint32_t* p, uint64_t nwords, uint32_t seed) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < nwords) p[i] = probe_word(i, seed);
+}
+
+// Counts the words of `dst` that differ from what probe `mode` leaves there
+// (`src` filled with seed `seed`); grid-stride, one atomic per mismatching wave.
+// `wrong` != 0 checks against a deliberately wrong expectation (the copy one
+// word off), so a test can show the check catches a probe that moved the
+// wrong data.
+__global__ __launch_bounds__(kBlock) void k_bw_verify(int mode, const uint32_t* src, const uint32_t* dst,
+                                                      uint64_t nwords, uint64_t cells, uint64_t nread_blk,
+                                                      uint32_t seed, uint32_t wrong, uint32_t* bad) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  const uint64_t total = mode == 1 ? nread_blk : (mode >= 5 ? 27 * cells : nwords);
+  uint32_t miss = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
+    uint32_t want;
+    uint64_t at = i;
+    if (mode == 1) {  // XOR of the 1024 words block i read
+      want = 0u;
+      for (uint64_t w = 0; w < uint64_t(4 * kBlock); ++w) {
+        const uint64_t k = i * 4 * kBlock + w;
+        if (k < nwords) want ^= probe_word(k, seed);
+      }
+    } else if (mode == 2) {
+      want = 0u;
+    } else if (mode >= 5) {
+      const uint64_t d = i / cells, c = i - d * cells;
+      const int cx = int(d % 3) - 1;
+      uint64_t cc = c;
+      if (mode == 6 && cx < 0) cc = c + 1 < cells ? c + 1 : c;
+      if (mode == 6 && cx > 0) cc = c > 0 ? c - 1 : c;
+      want = probe_word(d * cells + cc, seed);
+    } else {
+      if (mode == 4) at = i + 1;  // shifted by one element
+      want = probe_word(i, seed);
+    }
+    if (wrong) at += 1;
+    miss += dst[at] != want ? 1u : 0u;
+  }
+  const uint64_t ballot = __ballot(miss != 0u);
+  uint32_t sum = miss;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if (ballot && (threadIdx.x & 63) == 0) atomicAdd(bad, sum);
 }
 
 }  // namespace culbm

@@ -283,9 +283,15 @@ BW_MODES = {"copy": 0, "read": 1, "write": 2, "copy_scalar": 3, "copy_shifted": 
             "pop27_shifted": 6}
 
 
-def probe_bandwidth(mode: str, nbytes: int, iters: int = 20, device: int = 0) -> float:
-    """Achieved GB/s of a streaming probe over `nbytes` per buffer."""
+PROBE_CHECK_WRONG = 0x100
+
+
+def probe_bandwidth(mode: str, nbytes: int, iters: int = 20, device: int = 0, check_wrong: bool = False) -> float:
+    """Achieved GB/s of a streaming probe over `nbytes` per buffer.  The device
+    checks afterwards that the probe moved exactly the data it claims (raises
+    CulbmError otherwise); check_wrong=True checks against a deliberately wrong
+    expectation, which must raise."""
     g = ctypes.c_float()
-    _lib.check(_lib.lib().culbm_probe_bandwidth(device, BW_MODES[mode], int(nbytes), int(iters),
-                                                ctypes.byref(g)))
+    m = BW_MODES[mode] | (PROBE_CHECK_WRONG if check_wrong else 0)
+    _lib.check(_lib.lib().culbm_probe_bandwidth(device, m, int(nbytes), int(iters), ctypes.byref(g)))
     return g.value

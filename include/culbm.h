@@ -249,13 +249,29 @@ int culbm_probe_codec(int32_t device, int32_t storage, float scale, const float*
 int culbm_blocking_axis(int32_t dom, int32_t blk, int32_t iter, int32_t* num, int32_t* starts,
                         int32_t max_starts);
 
+/* Launch schedule of one temporal-blocking pass of the dynamic and static
+ * policies (the z-wavefront that replaces the reference's per-tile launch loop,
+ * simulator_expt_platform.cu:749-872): for a domain of nz planes, chunks of
+ * `depth` planes and `steps` steps, launches[3k..3k+2] = (z0, z1, t) of the
+ * k-th launch in issue order, step t (1-based) over planes [z0, z1).
+ * launches may be NULL to count only.  Host only (the hazard checker's input). */
+int culbm_wave_schedule(int32_t nz, int32_t depth, int32_t steps, int32_t* launches, int32_t max_launches,
+                        int32_t* count);
+
 /* Bandwidth calibration (counterpart of the reference's analysis/lambda and
- * analysis/gamma probes), one element per thread: mode 0 copy, 1 read,
- * 2 write (16-byte accesses), 3 scalar copy, 4 scalar copy with every store
- * shifted by one element, 5 the population access pattern (27 arrays, one
- * cell per thread, each value read and written once), 6 the same with the
- * pull gather's +-1 x offsets on 18 of the 27 loads.  `bytes` per buffer;
- * *gbs = bytes moved / time. */
+ * analysis/gamma probes, analysis/lambda/measure.cu:14-63,
+ * analysis/gamma/measure.cu:14-69), one element per thread: mode 0 copy,
+ * 1 read (each workgroup writes one XOR word of what it read), 2 write
+ * (16-byte accesses), 3 scalar copy, 4 scalar copy with every store shifted
+ * by one element, 5 the population access pattern (27 arrays, one cell per
+ * thread, each value read and written once), 6 the same with the pull
+ * gather's +-1 x offsets on 18 of the 27 loads.  `bytes` per buffer (4 KiB to
+ * 8 GiB); *gbs = bytes moved / time.  After the timed launches the device
+ * checks every word the probe claims to have moved (the source holds known
+ * words); a mismatch returns CULBM_ERR_STATE.  mode | CULBM_PROBE_CHECK_WRONG
+ * runs the same probe but checks it against a deliberately wrong expectation
+ * (one word off), which must fail: the test of the check itself. */
+#define CULBM_PROBE_CHECK_WRONG 0x100
 int culbm_probe_bandwidth(int32_t device, int32_t mode, int64_t bytes, int32_t iters, float* gbs);
 
 #ifdef __cplusplus

@@ -1,11 +1,18 @@
-"""The bandwidth probes behind culbm_amd.calibrate: every mode runs and reports
-a rate between a slow PCIe-class floor and twice MI355X's HBM peak, and the
-population-pattern probes (modes 5, 6) move the bytes they claim (their
-rate at a cache-resident size is at least their rate at an HBM-sized one)."""
+"""The bandwidth probes behind culbm_amd.calibrate.
+
+Every mode runs, reports a rate between a slow PCIe-class floor and twice
+MI355X's HBM peak, and passes the device-side data check: after the timed
+launches every word the probe claims to have moved (copied, shifted, gathered
+with the pull's x offsets, written, or read into its per-workgroup XOR) holds
+what it should.  The check itself is shown to catch a probe that moved the
+wrong data: run against a deliberately wrong expectation (one word off) it
+must fail.  The population-pattern probe is at least as fast from the cache
+as from HBM."""
 from __future__ import annotations
 
 import pytest
 
+from culbm_amd._lib import ERR_STATE, CulbmError
 from culbm_amd.lattice import BW_MODES, probe_bandwidth
 
 pytestmark = pytest.mark.gpu
@@ -14,9 +21,22 @@ MB = 1 << 20
 
 
 @pytest.mark.parametrize("mode", sorted(BW_MODES))
-def test_every_probe_reports_a_plausible_rate(mode):
-    g = probe_bandwidth(mode, 64 * MB, iters=3)
+def test_every_probe_moves_its_data_at_a_plausible_rate(mode):
+    g = probe_bandwidth(mode, 64 * MB, iters=3)  # raises if the data check fails
     assert 50.0 < g < 16000.0, (mode, g)
+
+
+@pytest.mark.parametrize("mode", sorted(BW_MODES))
+@pytest.mark.parametrize("nbytes", [64 * 1024 + 16, 3 * MB + 48])
+def test_data_check_on_ragged_sizes(mode, nbytes):
+    probe_bandwidth(mode, nbytes, iters=1)
+
+
+@pytest.mark.parametrize("mode", sorted(BW_MODES))
+def test_data_check_catches_wrong_data(mode):
+    with pytest.raises(CulbmError) as e:
+        probe_bandwidth(mode, 4 * MB, iters=1, check_wrong=True)
+    assert e.value.code == ERR_STATE and "moved wrong data" in e.value.message
 
 
 def test_population_pattern_is_faster_from_the_cache_than_from_hbm():

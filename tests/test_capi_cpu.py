@@ -141,7 +141,8 @@ class TestCli:
 
 
 @pytest.mark.parametrize("mode,nbytes,iters", [(7, 1 << 20, 1), (-1, 1 << 20, 1), (0, 100, 1), (5, 1 << 20, 0),
-                                               (6, (64 << 30) + 16, 1)])
+                                               (6, (64 << 30) + 16, 1), (3, (8 << 30) + 16, 1),
+                                               (5, 16 << 10, 1), (0x200, 1 << 20, 1)])
 def test_bandwidth_probe_rejects_bad_arguments_before_touching_a_device(mode, nbytes, iters):
     g = ctypes.c_float()
     rc = _lib.lib().culbm_probe_bandwidth(0, mode, nbytes, iters, ctypes.byref(g))

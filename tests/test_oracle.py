@@ -227,17 +227,32 @@ def test_negative_zero_is_stored_as_positive_zero_in_fp16():
     assert enc[0] == 0
 
 
-def test_bf16_codec_round_to_nearest_even():
+def test_bf16_codec_matches_torch_round_to_nearest_even():
+    """BF16 encode against torch's float32 -> bfloat16 conversion (round to
+    nearest, ties to even), an implementation independent of the oracle's.  The
+    inputs include exact ties (the 16 dropped bits = 0x8000) with both an even
+    and an odd kept bit, values one ulp either side of a tie, and the carry
+    into the exponent."""
+    torch = pytest.importorskip("torch")
     rng = np.random.default_rng(10)
     x = (rng.normal(0, 1, 100000) * 10.0 ** rng.integers(-20, 20, 100000)).astype(np.float32)
+    hi = rng.integers(0, 1 << 16, 20000).astype(np.uint32) << 16
+    hi &= np.uint32(0xBF7FFFFF)  # exponent bits 30 and 23 cleared: finite values
+    ties = np.concatenate([hi | 0x8000, hi | 0x7FFF, hi | 0x8001, np.array([0x3F7FFFFF, 0x3F7F8000, 0x3F808000,
+                                                                              0x3F818000, 0x7F7F7FFF], np.uint32)])
+    x = np.concatenate([x, ties.view(np.float32)])
+    x = x[np.isfinite(x)]
     enc = np.zeros(x.size, np.uint16)
     po.lib().orc_encode16(2, x, enc, x.size)
-    u = x.view(np.uint32).astype(np.uint64)
-    want = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
-    assert np.array_equal(enc, want)
+    want = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(enc, want), np.flatnonzero(enc != want)[:5]
+    # ties really occurred, on both sides
+    tie = (x.view(np.uint32) & 0xFFFF) == 0x8000
+    assert tie.sum() > 1000 and len(set((x.view(np.uint32)[tie] >> 16) & 1)) == 2
+    # decode is exact: the code's 16 bits are the upper half of the FP32 value
     dec = np.zeros(x.size, np.float32)
     po.lib().orc_decode16(2, enc, dec, x.size)
-    assert np.all(np.abs(dec - x) <= np.abs(x) * 2.0 ** -8)
+    assert np.array_equal(dec, torch.from_numpy(enc.view(np.int16)).view(torch.bfloat16).float().numpy())
 
 
 # ---- stochastic rounding of the 16-bit formats ---------------------------------
