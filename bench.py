@@ -8,10 +8,12 @@ invTau 0.5, in-place streaming + temporal blocking with innerLoop 5.  One
 "step" is one lattice update of all 21,934,080 cells; the state is resident
 in HBM before timing starts.
 
-Prints ONE JSON line: MLUPS of the headline form (value), the HBM roofline of
-its dominant kernel (216 algorithmic bytes per FP32 cell-update, 108 for
-16-bit storage), the other forms measured in the same run, and the FP32
-OpenMP CPU restatement timed on a bounded sample of the same workload.
+Prints ONE JSON line: MLUPS of the headline form (value: the median of five
+device-timed 20-step windows after a warm-up window), the HBM roofline of its
+dominant kernel (216 algorithmic bytes per FP32 cell-update, 108 for 16-bit
+storage), the other forms measured in the same run, the FP16 form's speed
+ratio and its error against the FP64 oracle, and the FP32 OpenMP CPU
+restatement timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -60,53 +62,58 @@ def _sync():
         pass
 
 
-def run_form(state, name, steps, warmup, inner_loop):
+def run_form(state, name, steps, warmup, inner_loop, windows=5, window_steps=20):
+    """One form on `state`: `warmup` untimed steps, then exactly `steps` steps
+    timed (wall clock with device syncs on both sides, and an event pair), then
+    the reference's measurement (simulator_expt_platform.cu:638-659, the survey's
+    measurement rule): one warm-up window and `windows` device-timed windows of
+    `window_steps` steps; MLUPS is their median."""
     from culbm_amd import Lattice, LatticeConfig
 
     sp, op, storage = FORMS[name][:3]
     rounding = FORMS[name][4] if len(FORMS[name]) > 4 else "nearest"
-    lat = Lattice(LatticeConfig(domain=DOMAIN, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
+    domain = state.domain
+    lat = Lattice(LatticeConfig(domain=domain, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
                                 inner_loop=inner_loop, storage=storage, rounding=rounding))
     lat.set_state(state.flag, state.rho, state.vx, state.vy, state.vz)
     lat.init_ddf("rest")
     lat.step(warmup)
     _sync()
     t0 = time.perf_counter()
-    kernel_ms = lat.time_steps(steps)  # event pair around exactly `steps` steps
+    region_ms = lat.time_steps(steps)  # event pair around exactly `steps` steps
     _sync()
     wall = time.perf_counter() - t0
     info = lat.info()
     prof = lat.profile_steps(min(steps, 50))  # per-launch events: kernel time only
-    # the reference's measurement: device-timed windows, one warm-up window, then the median
-    lat.run_window(20)
-    ncell_w = DOMAIN[0] * DOMAIN[1] * DOMAIN[2]
-    windows = []
-    for _ in range(5):
-        done, ms = lat.run_window(20)
-        windows.append(ncell_w * done / (ms * 1e-3) / 1e6)
+    ncell = domain[0] * domain[1] * domain[2]
+    lat.run_window(window_steps)  # warm-up window
+    win = []
+    for _ in range(windows):
+        done, ms = lat.run_window(window_steps)
+        win.append((ncell * done / (ms * 1e-3) / 1e6, ms, done))
     lat.close()
-    ncell = DOMAIN[0] * DOMAIN[1] * DOMAIN[2]
+    med = sorted(win)[len(win) // 2]  # the median window (odd count)
+    mlups, win_ms, win_steps = med
     bpu = 27 * 2 * info["bytes_per_pop"]  # algorithmic bytes per cell-update
-    mlups = ncell * steps / wall / 1e6
-    k_gbs = bpu * ncell * prof["steps"] / (prof["kernel_ms"] * 1e-3) / 1e9
-    launches = prof["launches"] / prof["steps"] * steps
+    launches_per_step = prof["launches"] / prof["steps"]
     return {
         "mlups": mlups,
+        "window_mlups": [round(w[0], 1) for w in win],
+        "window_steps": win_steps,
+        "region_mlups": ncell * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
-        "event_ms_per_step": kernel_ms / steps,
+        "event_ms_per_step": region_ms / steps,
         "bytes_per_update": bpu,
         "effective_gbs": mlups * 1e6 * bpu / 1e9,
-        "kernel_gbs": k_gbs,
+        # average launch of the median window (its event pair over its launch
+        # count: one kernel per form except mixed, whose windows also hold the
+        # odd pull steps) and the per-launch event figure (adds ~2.7 us per launch)
+        "window_launch_avg_us": win_ms * 1e3 / (launches_per_step * win_steps),
+        "kernel_gbs": bpu * ncell * prof["steps"] / (prof["kernel_ms"] * 1e-3) / 1e9,
         "kernel_avg_us": prof["kernel_ms"] * 1e3 / prof["launches"],
-        # the event pair around the timed region over its launch count: the
-        # average launch duration without per-launch event overhead (one kernel
-        # per form except mixed, whose region also holds the odd pull steps)
-        "timed_launch_avg_us": kernel_ms * 1e3 / launches,
-        "timed_kernel_gbs": bpu * ncell * steps / (kernel_ms * 1e-3) / 1e9,
-        "launches_per_step": prof["launches"] / prof["steps"],
+        "launches_per_step": launches_per_step,
         "wave_depth": info["wave_depth"],
         "frac_hbm": mlups * 1e6 * bpu / 1e9 / HBM_PEAK_GBS,
-        "window_median_mlups": float(np.median(windows)),
     }
 
 
@@ -242,6 +249,52 @@ def precision_report(n=512, steps=500):
     return out
 
 
+FP16_CLAIM = "fp16_sr_inplace_blocked"  # the 16-bit form the >= 1.7x at <= 1e-3 target is claimed for
+PREC_N, PREC_STEPS = 128, 500
+
+
+def fp16_claim_gpu(forms, head, inner_loop):
+    """The FP16 form's speed against the FP32 headline (both window medians of
+    this run) and its rho/u after the precision workload at 128^3, 500 steps
+    (zero-gradient channel, rho = 1 + U(-1e-3, 1e-3) from seed 3, populations at
+    feq, in place + dynamic blocking) for the error against FP64."""
+    from culbm_amd import Lattice, LatticeConfig, cases
+
+    sp, op, storage = FORMS[FP16_CLAIM][:3]
+    st = cases.config_precision(PREC_N, seed=3)
+    with Lattice(LatticeConfig(domain=st.domain, inv_tau=INV_TAU, stream_policy=sp, opt_policy=op,
+                               inner_loop=inner_loop, storage=storage, rounding=FORMS[FP16_CLAIM][4])) as lat:
+        lat.set_state(st.flag, st.rho, st.vx, st.vy, st.vz)
+        lat.init_ddf("feq")
+        lat.step(PREC_STEPS)
+        lat.dump()
+        macros = lat.macros()
+    mlups = forms[FP16_CLAIM]["mlups"]
+    return {
+        "form": FP16_CLAIM,
+        "mlups": round(mlups, 1),
+        "ratio_to_headline": round(mlups / head["mlups"], 4),
+        "error_workload": f"{PREC_N}^3 zero-gradient channel, rho = 1 + U(-1e-3, 1e-3), u_in 0.05, "
+                          f"invTau 0.5, populations at feq, {PREC_STEPS} steps, fluid cells",
+    }, st, macros
+
+
+def fp16_claim_error(st, macros):
+    """rho and u error of the FP16 form's run against the FP64 oracle on the
+    host (the checker; part of the CPU leg)."""
+    from oracle import pyoracle
+
+    sim = pyoracle.Sim(st, INV_TAU, np.float64)
+    sim.init_feq()
+    sim.step(PREC_STEPS)
+    ref = sim.dump()
+    fluid = (st.flag & (1 << 27)) != 0
+    d_rho = np.abs(macros[0].astype(np.float64) - ref[0])[fluid]
+    d_u = np.sqrt(sum((g.astype(np.float64) - r) ** 2 for g, r in zip(macros[1:], ref[1:])))[fluid] / 0.05
+    return {"rho_max": float(d_rho.max()), "rho_rms": float(np.sqrt((d_rho ** 2).mean())),
+            "u_max_over_u_in": float(d_u.max()), "u_rms_over_u_in": float(np.sqrt((d_u ** 2).mean()))}
+
+
 def cpu_baseline(state, budget_s=12.0):
     """FP32 OpenMP restatement of the plain pull step on the same workload."""
     os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime loads
@@ -277,6 +330,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--form", default=HEADLINE, choices=sorted(FORMS))
     ap.add_argument("--inner-loop", type=int, default=5)
+    ap.add_argument("--windows", type=int, default=5, help="timed windows (odd; value = their median)")
+    ap.add_argument("--window-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="measure only the headline form")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
@@ -285,6 +340,8 @@ def main(argv=None) -> int:
     ap.add_argument("--adjoint-large", action="store_true",
                     help="also run the 256^3 FP16 adjoint with binomial checkpoints")
     ns = ap.parse_args(argv)
+    if ns.windows < 5 or ns.windows % 2 == 0:
+        ap.error("--windows must be odd and at least 5")
     if ns.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not provided "
                                    "(a z-slab decomposition with one 9-direction face exchange per step over xGMI "
@@ -294,15 +351,20 @@ def main(argv=None) -> int:
     from culbm_amd import cases
 
     state = cases.config_obstacles(*DOMAIN, count=60, r_min=6.0, r_max=14.0, seed=2)
-    head = run_form(state, ns.form, ns.steps, ns.warmup, ns.inner_loop)
+    head = run_form(state, ns.form, ns.steps, ns.warmup, ns.inner_loop, ns.windows, ns.window_steps)
+    print(json.dumps({"headline_progress": round(head["mlups"], 1)}), file=sys.stderr, flush=True)
     forms = {}
     if not ns.no_forms:
         for name in FORMS:
             if name == ns.form:
                 continue
-            r = run_form(state, name, min(ns.steps, 50), min(ns.warmup, 5), ns.inner_loop)
+            r = run_form(state, name, min(ns.steps, 50), min(ns.warmup, 5), ns.inner_loop, ns.windows,
+                         ns.window_steps)
             forms[name] = {k: round(v, 4) if isinstance(v, float) else v for k, v in r.items()}
         print(json.dumps({"forms_progress": "done"}), file=sys.stderr, flush=True)
+    claim, claim_state, claim_macros = (None, None, None)
+    if FP16_CLAIM in forms:
+        claim, claim_state, claim_macros = fp16_claim_gpu(forms, head, ns.inner_loop)
     adj = None
     if not ns.no_adjoint:
         adj = adjoint_ratio(ns.adjoint_large)
@@ -310,14 +372,21 @@ def main(argv=None) -> int:
     prec = precision_report() if ns.precision else None
     cpu = None
     if not ns.no_cpu:
+        # the CPU leg: the FP32 OpenMP baseline, and the FP64 oracle as the
+        # checker of the FP16 form's error
         cpu = cpu_baseline(state)
+        if claim is not None:
+            claim["error_vs_fp64"] = fp16_claim_error(claim_state, claim_macros)
+            e = claim["error_vs_fp64"]
+            claim["meets_target"] = bool(claim["ratio_to_headline"] >= 1.7 and e["rho_max"] <= 1e-3
+                                         and e["u_max_over_u_in"] <= 1e-3)
 
     # dominant kernel's rate: algorithmic bytes per launch over the average launch
-    # duration from the event pair around the timed region (single-kernel forms);
-    # the mixed form's region also holds k_pull, so it takes per-launch events
+    # duration of the median window (its event pair over its launch count; the
+    # mixed form's windows also hold k_pull, so it takes per-launch events)
     single = FORMS[ns.form][1] != 2
-    achieved = head["timed_kernel_gbs"] if single else head["kernel_gbs"]
-    avg_us = head["timed_launch_avg_us"] if single else head["kernel_avg_us"]
+    achieved = head["effective_gbs"] if single else head["kernel_gbs"]
+    avg_us = head["window_launch_avg_us"] if single else head["kernel_avg_us"]
     # HBM bytes per launch of the dominant kernel: the counter-derived bytes per
     # cell-update committed under profiles/ times the cells one launch updates
     traffic = None
@@ -351,6 +420,11 @@ def main(argv=None) -> int:
             "inner_loop": ns.inner_loop,
             "wave_depth": head["wave_depth"],
         },
+        "measurement": (f"value: median of {ns.windows} device-timed windows of {head['window_steps']} steps "
+                        f"after one warm-up window; ms_per_step: wall clock of the {ns.steps}-step timed "
+                        f"region with device syncs on both sides"),
+        "window_mlups": head["window_mlups"],
+        "region_mlups": round(head["region_mlups"], 1),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -364,7 +438,7 @@ def main(argv=None) -> int:
             "bytes_per_update": head["bytes_per_update"],
         },
         "effective_frac_hbm": round(head["frac_hbm"], 4),
-        "window_median_mlups": round(head["window_median_mlups"], 1),
+        "fp16_claim": claim,
         "adjoint_forward_ratio": adj,
         "precision": prec,
         "forms": forms,

@@ -109,6 +109,7 @@ SIGNATURES = {
     "culbm_time_steps": (ctypes.c_int, [_VP, _I32, ctypes.POINTER(ctypes.c_float)]),
     "culbm_profile_steps": (ctypes.c_int, [_VP, _I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I32)]),
     "culbm_dump": (ctypes.c_int, [_VP]),
+    "culbm_dump_deferred": (ctypes.c_int, [_VP]),
     "culbm_get_macros": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "culbm_get_ddf": (ctypes.c_int, [_VP, _F32]),
     "culbm_set_ddf": (ctypes.c_int, [_VP, _F32]),

@@ -173,13 +173,28 @@ def config_adjoint(n: int = 128, seed: int = 4, count: int = 20, r_min: float = 
     return st, prof
 
 
+def config_correctness(nx: int = 768, ny: int = 384, nz: int = 384, cube: Tuple[int, int] = (128, 256),
+                       u_in: float = 0.05) -> State:
+    """The reference's own bitwise correctness case (analysis/correctness/
+    generate.py:6-42): the channel convention (equilibrium outlet) at
+    768x384x384 with a solid cube over [128, 256) on every axis.  Its case
+    folder holds flag.dat and vx.dat only (rho, vy, vz take the defaults)."""
+    st = channel(nx, ny, nz, u_in=u_in, outlet="equilibrium")
+    a, b = cube
+    st.flag[a:b, a:b, a:b] = WALL
+    st.vx[a:b, a:b, a:b] = 0.0
+    return st
+
+
 # ---- files -----------------------------------------------------------------
 
-def write_state(folder, state: State) -> None:
+def write_state(folder, state: State, fields=FIELDS) -> None:
+    """flag.dat plus the named fields (a field left out takes its default when
+    read back, as a missing file does for the reference)."""
     folder = Path(folder)
     folder.mkdir(parents=True, exist_ok=True)
     state.flag.astype("<u4").tofile(folder / "flag.dat")
-    for name in FIELDS:
+    for name in fields:
         getattr(state, name).astype("<f4").tofile(folder / f"{name}.dat")
 
 

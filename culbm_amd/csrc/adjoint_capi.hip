@@ -490,6 +490,7 @@ int culbm_adjoint(culbm_ctx* ctx, const culbm_adjoint_spec* spec, const double* 
   if (size_t(27) * size_t(ctx->n) * 4 >= (size_t(1) << 32))
     return fail(ctx, CULBM_ERR_UNSUPPORTED, "adjoint buffers need 27*4*cells < 2^32 bytes");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  if (int rc = flush_dump(ctx)) return rc;
   double* g[4] = {grad_rho, grad_ux, grad_uy, grad_uz};
   return by_storage(ctx->cfg.storage, [&](auto st) {
     return adjoint_impl<decltype(st)>(ctx, s, weight, stats, g, s.want_f0 ? grad_f0 : nullptr);

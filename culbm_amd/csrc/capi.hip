@@ -46,6 +46,9 @@ struct culbm_ctx {
   // (pull: index of the current buffer; in-place: parity of the next step)
   hipGraphExec_t pass_graph[2] = {nullptr, nullptr};
   int64_t device_bytes = 0;
+  // a dump armed by culbm_dump_deferred: the next step writes it (fused), or
+  // the next call that reads or replaces the fields or populations runs k_dump
+  bool dump_pending = false;
   // per-launch timing (culbm_profile_steps): an event pair around every launch
   bool prof = false;
   size_t prof_used = 0;
@@ -271,6 +274,12 @@ StepArgs base_args(const culbm_ctx* ctx) {
   a.omt = 1.0f - ctx->cfg.inv_tau;
   a.scale = ctx->scale;
   a.inv_scale = ctx->inv_scale;
+  a.neg_zero = -0.0f;
+  a.flag = ctx->flag;
+  a.mrho = ctx->rho;
+  a.mvx = ctx->vx;
+  a.mvy = ctx->vy;
+  a.mvz = ctx->vz;
   return a;
 }
 
@@ -317,20 +326,29 @@ int prof_end(culbm_ctx* ctx) {
 }
 
 // One step over planes [z0, z1); a.state is the number of the state it produces.
-template <class S, bool SR>
+// DUMP: the launch also writes the pending dump of the planes it covers.
+template <class S, bool SR, bool DUMP>
 void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity) {
   if (ctx->inplace) {
     if (parity)
-      hipLaunchKernelGGL((k_twist<S, 1, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_twist<S, 1, SR, DUMP>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
     else
-      hipLaunchKernelGGL((k_twist<S, 0, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_twist<S, 0, SR, DUMP>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
   } else {
-    hipLaunchKernelGGL((k_pull<S, SR>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_pull<S, SR, DUMP>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
   }
 }
 
+template <class S, bool SR>
+void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity, bool dump) {
+  if (dump)
+    launch_step_kernel<S, SR, true>(ctx, a, parity);
+  else
+    launch_step_kernel<S, SR, false>(ctx, a, parity);
+}
+
 template <class S>
-int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) {
+int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity, bool dump = false) {
   set_range(a, z0, z1, !ctx->inplace, ctx->bpp == 2);
   if (a.ncells == 0) return CULBM_OK;
   if (int rc = prof_begin(ctx)) return rc;
@@ -340,11 +358,11 @@ int launch_step(culbm_ctx* ctx, StepArgs a, int32_t z0, int32_t z1, int parity) 
   }
   if constexpr (S::kBytes == 2) {
     if (ctx->sr)
-      launch_step_kernel<S, true>(ctx, a, parity);
+      launch_step_kernel<S, true>(ctx, a, parity, dump);
     else
-      launch_step_kernel<S, false>(ctx, a, parity);
+      launch_step_kernel<S, false>(ctx, a, parity, dump);
   } else {
-    launch_step_kernel<S, false>(ctx, a, parity);
+    launch_step_kernel<S, false>(ctx, a, parity, dump);
   }
   HIPCHK(ctx, hipGetLastError());
   return prof_end(ctx);
@@ -375,15 +393,18 @@ template <class S>
 int wave_pass(culbm_ctx* ctx, int32_t T) {
   StepArgs a = base_args(ctx);
   const uint32_t base = uint32_t(ctx->step);
+  // a pending dump rides on the first step: its launches cover every plane once
+  const bool dump = ctx->dump_pending;
   const int rc = for_each_wave_launch(ctx->cfg.nz, ctx->wave_depth, T, [&](int32_t lo, int32_t hi, int32_t t) {
     if (!ctx->inplace) {
       a.src = ctx->buf[(ctx->cur + t - 1) & 1];
       a.dst = ctx->buf[(ctx->cur + t) & 1];
     }
     a.state = base + uint32_t(t);
-    return launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1);
+    return launch_step<S>(ctx, a, lo, hi, (ctx->parity + t - 1) & 1, dump && t == 1);
   });
   if (rc) return rc;
+  ctx->dump_pending = false;
   ctx->cur = (ctx->cur + T) & 1;
   ctx->parity = (ctx->parity + T) & 1;
   ctx->step += T;
@@ -397,7 +418,7 @@ int wave_pass(culbm_ctx* ctx, int32_t T) {
 // every pass with stochastic rounding (each launch carries its step number).
 template <class S>
 int static_pass(culbm_ctx* ctx, int32_t T) {
-  if (T != ctx->cfg.inner_loop || ctx->sr) return wave_pass<S>(ctx, T);
+  if (T != ctx->cfg.inner_loop || ctx->sr || ctx->dump_pending) return wave_pass<S>(ctx, T);
   const int key = ctx->inplace ? ctx->parity : ctx->cur;
   if (!ctx->pass_graph[key]) {
     const int cur = ctx->cur, parity = ctx->parity;
@@ -473,8 +494,16 @@ int fused_pass(culbm_ctx* ctx, int32_t T) {
 }
 
 template <class S>
+int dump_impl(culbm_ctx* ctx);
+
+template <class S>
 int do_steps(culbm_ctx* ctx, int32_t nsteps) {
   const culbm_config& c = ctx->cfg;
+  if (nsteps > 0 && ctx->dump_pending && c.opt_policy == CULBM_OPT_MIXED) {
+    // the two-step kernel gathers tiles with halos: the dump runs as its own pass
+    if (int rc = dump_impl<S>(ctx)) return rc;
+    ctx->dump_pending = false;
+  }
   if (!blocked(c)) {
     StepArgs a = base_args(ctx);
     for (int32_t s = 0; s < nsteps; ++s) {
@@ -483,8 +512,9 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
         a.dst = ctx->buf[ctx->cur ^ 1];
       }
       a.state = uint32_t(ctx->step) + 1u;
-      const int rc = launch_step<S>(ctx, a, 0, c.nz, ctx->parity);
+      const int rc = launch_step<S>(ctx, a, 0, c.nz, ctx->parity, ctx->dump_pending);
       if (rc) return rc;
+      ctx->dump_pending = false;
       ctx->cur ^= 1;
       ctx->parity ^= 1;
       ctx->step += 1;
@@ -584,6 +614,16 @@ int by_storage(int32_t storage, F&& f) {
     case CULBM_STORAGE_BF16: return f(StoreBF16{});
     default: return f(StoreF32{});
   }
+}
+
+// Run a dump armed by culbm_dump_deferred that no step has taken yet (as a
+// separate pass), before a call reads or replaces the fields or populations.
+int flush_dump(culbm_ctx* ctx) {
+  if (!ctx->dump_pending) return CULBM_OK;
+  const int rc = by_storage(ctx->cfg.storage, [&](auto s) { return dump_impl<decltype(s)>(ctx); });
+  if (rc) return rc;
+  ctx->dump_pending = false;
+  return CULBM_OK;
 }
 
 int zero_device(culbm_ctx* ctx, void* p, size_t bytes) {
@@ -787,6 +827,7 @@ int culbm_set_state(culbm_ctx* ctx, const uint32_t* flag, const float* rho, cons
                     const float* vy, const float* vz) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  if (int rc = flush_dump(ctx)) return rc;
   const size_t nb = size_t(ctx->n);
   if (flag) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->flag, flag, nb * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -827,6 +868,7 @@ int culbm_init_ddf(culbm_ctx* ctx, int32_t mode) {
   if (mode != CULBM_INIT_REST && mode != CULBM_INIT_FEQ)
     return fail(ctx, CULBM_ERR_INVALID, "init mode must be 0 (rest) or 1 (feq), got %d", mode);
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  if (int rc = flush_dump(ctx)) return rc;
   for (int b = 0; b < 2; ++b)
     if (ctx->buf[b]) {
       const int rc = zero_device(ctx, ctx->buf[b], ddf_bytes(ctx));
@@ -914,15 +956,23 @@ int culbm_run_window(culbm_ctx* ctx, int32_t dstep, int32_t* steps_done, float* 
 int culbm_dump(culbm_ctx* ctx) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  ctx->dump_pending = false;  // this pass delivers it
   const int rc = by_storage(ctx->cfg.storage, [&](auto s) { return dump_impl<decltype(s)>(ctx); });
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CULBM_OK;
 }
 
+int culbm_dump_deferred(culbm_ctx* ctx) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  ctx->dump_pending = true;
+  return CULBM_OK;
+}
+
 int culbm_get_macros(culbm_ctx* ctx, float* rho, float* vx, float* vy, float* vz) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  if (int rc = flush_dump(ctx)) return rc;
   float* dst[4] = {rho, vx, vy, vz};
   const float* src[4] = {ctx->rho, ctx->vx, ctx->vy, ctx->vz};
   for (int k = 0; k < 4; ++k)
@@ -952,6 +1002,7 @@ int culbm_set_ddf(culbm_ctx* ctx, const float* ddf) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   if (!ddf) return fail(ctx, CULBM_ERR_INVALID, "ddf is NULL");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  if (int rc = flush_dump(ctx)) return rc;
   const size_t bytes = size_t(27) * size_t(ctx->n) * 4;
   float* plain = nullptr;
   HIPCHK(ctx, hipMalloc(&plain, bytes));

@@ -107,13 +107,13 @@ struct PullGather {
   int64_t dsb;
   uint32_t o[3][3];  // byte offset of the clamped source row (y - cy, z - cz), [cy+1][cz+1]
   int32_t xm, xp;    // byte offset to x-1 / x+1 (0 when clamped on that face)
-  float inv_s;
+  float inv_s, nz;
 
   template <int I>
   __device__ __forceinline__ void run(float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     const int32_t ox = cx == 1 ? xm : (cx == -1 ? xp : 0);  // source x - c_x
-    f[I] = S::dec(ld_at<typename S::T>(base + I * dsb, o[cy + 1][cz + 1] + uint32_t(ox)), inv_s);
+    f[I] = S::dec_fast(ld_at<typename S::T>(base + I * dsb, o[cy + 1][cz + 1] + uint32_t(ox)), inv_s, nz);
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
@@ -137,6 +137,7 @@ __device__ __forceinline__ PullGather<S> make_gather(const StepArgs& a, const Ce
   g.xm = k.x == 0 ? 0 : -B;
   g.xp = k.x == a.nx - 1 ? 0 : B;
   g.inv_s = a.inv_scale;
+  g.nz = a.neg_zero;
   return g;
 }
 
@@ -209,24 +210,61 @@ __device__ __forceinline__ EquIn equ_prefetch(const StepArgs& a, uint32_t c, int
 }
 
 // The per-cell update shared by every variant: LOAD -> EQU -> BB -> COLLIDE
-// (kernel.cu:2501-2631).  f holds the loaded values on entry.
+// (kernel.cu:2501-2631).  f holds the loaded values on entry (for every cell:
+// the loads are issued before the flag arrives).  A cell with EQU replaces
+// them by feq(rho, u); a cell with neither LOAD nor EQU starts from zero.  Both
+// replacements sit in one divergent block that only waves holding such a cell
+// enter, so interior waves pay no per-value selects for the zeroing.
 __device__ __forceinline__ void update_cell(uint32_t fl, const StepArgs& a, uint32_t c, const EquIn& q,
                                             float* f) {
-  if (fl & kEqu) {
-    float rho = q.rho, vx = q.vx, vy = q.vy, vz = q.vz;
-    if (!q.ok) {  // an EQU cell off the x faces
-      rho = a.rho[c];
-      vx = a.vx[c];
-      vy = a.vy[c];
-      vz = a.vz[c];
+  if ((fl & kEqu) || !(fl & kLoad)) {
+    if (fl & kEqu) {
+      float rho = q.rho, vx = q.vx, vy = q.vy, vz = q.vz;
+      if (!q.ok) {  // an EQU cell off the x faces
+        rho = a.rho[c];
+        vx = a.vx[c];
+        vy = a.vy[c];
+        vz = a.vz[c];
+      }
+      equilibrium(rho, vx, vy, vz, f);
+    } else {
+      clear(f);
     }
-    equilibrium(rho, vx, vy, vz, f);
   }
   if (fl & kBounceBack) bounce_back(f);
   if (fl & kCollide) collide(a.inv_tau, a.omt, f);
 }
 
-template <class S, bool SR>
+// Fused dump: the reference's dump pass (StaticD3Q27PullDumpKernel,
+// kernel.cu:2634-2735) riding on the first step after a window.  The step has
+// just gathered exactly the populations the dump pass would (pull(src), or the
+// twisted loads, for every cell whatever its flag), so it writes their rho/u
+// where the DUMP bits are set, as the dump pass does.  A dumped field is also
+// that cell's EQU input from then on (the two share one buffer, as in the
+// reference), so an EQU cell takes the dumped values in this very step.
+__device__ __forceinline__ void fused_dump(const StepArgs& a, uint32_t c, const float* f, EquIn& q) {
+  const uint32_t fw = a.flag[c];
+  if (!(fw & (kDumpRho | kDumpVx | kDumpVy | kDumpVz))) return;
+  const Moments m = moments(f);
+  if (fw & kDumpRho) {
+    a.mrho[c] = m.rho;
+    q.rho = m.rho;
+  }
+  if (fw & kDumpVx) {
+    a.mvx[c] = m.ux;
+    q.vx = m.ux;
+  }
+  if (fw & kDumpVy) {
+    a.mvy[c] = m.uy;
+    q.vy = m.uy;
+  }
+  if (fw & kDumpVz) {
+    a.mvz[c] = m.uz;
+    q.vz = m.uz;
+  }
+}
+
+template <class S, bool SR, bool DUMP = false>
 __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   const Cell k = locate(a, cell_block(a));
   if (!k.ok) return;
@@ -234,9 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_pull(const StepArgs a) {
   // the flag arrives instead of waiting on it.
   float f[27];
   make_gather<S>(a, k, a.src).template run<0>(f);
-  const EquIn q = equ_prefetch(a, k.c, k.x);
+  EquIn q = equ_prefetch(a, k.c, k.x);
+  if constexpr (DUMP) fused_dump(a, k.c, f, q);
   const uint32_t fl = a.sflag[k.c];
-  if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, q, f);
   if (fl & kStore) store_plain_all<S, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
@@ -270,13 +308,13 @@ struct TwistLoad {
   const char* base;
   int64_t dsb;
   TwistRows r;
-  float inv_s;
+  float inv_s, nz;
   template <int I>
   __device__ __forceinline__ void run(float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? I : 26 - I;
     constexpr uint32_t ox = cx == -1 ? sizeof(typename S::T) : 0;
-    f[I] = S::dec(ld_at<typename S::T>(base + slot * dsb, r.o[cy == -1][cz == -1] + ox), inv_s);
+    f[I] = S::dec_fast(ld_at<typename S::T>(base + slot * dsb, r.o[cy == -1][cz == -1] + ox), inv_s, nz);
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
@@ -361,6 +399,7 @@ __device__ __forceinline__ void twist_load_all(const StepArgs& a, const Cell& k,
   t.dsb = a.dstride * int64_t(sizeof(typename S::T));
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
   t.inv_s = a.inv_scale;
+  t.nz = a.neg_zero;
   t.template run<0>(f);
 }
 
@@ -395,7 +434,7 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   }
 }
 
-template <class S, int P, bool SR>
+template <class S, int P, bool SR, bool DUMP = false>
 __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   const Cell k = locate(a, cell_block(a));
   if (!k.ok) return;
@@ -403,9 +442,9 @@ __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   // before the flag arrives
   float f[27];
   twist_load_all<S, P>(a, k, a.dst, f);
-  const EquIn q = equ_prefetch(a, k.c, k.x);
+  EquIn q = equ_prefetch(a, k.c, k.x);
+  if constexpr (DUMP) fused_dump(a, k.c, f, q);
   const uint32_t fl = a.sflag[k.c];
-  if (!(fl & kLoad)) clear(f);
   update_cell(fl, a, k.c, q, f);
   if (fl & kStore) twist_store_all<S, P, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
@@ -505,7 +544,6 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
     qe.vy = a.vy[q.c];
     qe.vz = a.vz[q.c];
   }
-  if (!(q.fl & kLoad)) clear(q.f);
   update_cell(q.fl, a, q.c, qe, q.f);
   T v[27];
   if (q.fl & kStore) {
@@ -636,7 +674,6 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
       e.vx = q4[1];
       e.vy = q4[2];
       e.vz = q4[3];
-      if (!(fl & kLoad)) clear(f);
       update_cell(fl, fa.a, k.c, e, f);
       if (fl & kStore)  // the second step produces state a.state + 1
         store_plain_all<S, SR>(fa.a, k, fa.a.dst, f, make_encoder<S, SR>(fa.a, k.c, fa.a.state + 1));

@@ -68,6 +68,7 @@ struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
   __device__ __forceinline__ static float dec(T v, float) { return v; }
+  __device__ __forceinline__ static float dec_fast(T v, float, float) { return v; }
   __device__ __forceinline__ static T enc(float v, float) { return v; }
   __device__ __forceinline__ static T enc_sr(float v, float, uint32_t) { return v; }
 };
@@ -76,10 +77,16 @@ struct StoreF16 {
   using T = _Float16;
   static constexpr int kBytes = 2;
   __device__ __forceinline__ static float dec(T v, float inv_s) { return float(v) * inv_s; }
-  // `+ 0.0f` maps -0 to +0 before rounding: the compiler may fold the scale
-  // and the conversion into one mixed-precision FMA with a +0 addend, which
-  // does the same; writing it out makes every code path agree.
-  __device__ __forceinline__ static T enc(float v, float s) { return T(v * s + 0.0f); }
+  // The same value as one mixed-precision FMA (v_fma_mix_f32): the product of a
+  // binary16 value and a power of two is exact, and a -0 addend leaves every
+  // product unchanged, the sign of a zero included.  nz must be -0.0f; it comes
+  // in as a kernel argument, since a -0 constant lets the compiler fold the FMA
+  // back into a conversion and a multiply.
+  __device__ __forceinline__ static float dec_fast(T v, float inv_s, float nz) { return fmaf(float(v), inv_s, nz); }
+  // round-to-nearest-even(v * s + 0) in one mixed-precision FMA with a binary16
+  // result (v_fma_mixlo_f16): v * s is exact, so the single rounding equals the
+  // conversion of v * s; the +0 addend maps -0 to +0.
+  __device__ __forceinline__ static T enc(float v, float s) { return T(fmaf(v, s, 0.0f)); }
   // stochastic: add b * 2^5 + 2^4 below the 10 kept mantissa bits of the FP32
   // value, clear the 13 dropped bits, convert (exact in the binary16 normal
   // range; below it the conversion rounds to the subnormal grid)
@@ -95,10 +102,11 @@ struct StoreBF16 {
   __device__ __forceinline__ static float dec(T v, float inv_s) {
     return __uint_as_float(uint32_t(v) << 16) * inv_s;
   }
+  __device__ __forceinline__ static float dec_fast(T v, float inv_s, float) { return dec(v, inv_s); }
+  // round to nearest even by the hardware conversion (v_cvt_pk_bf16_f32), the
+  // same bits as adding 0x7fff + (bit 16) to the FP32 pattern and truncating
   __device__ __forceinline__ static T enc(float v, float s) {
-    uint32_t u = __float_as_uint(v * s);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return T(u >> 16);
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v * s));
   }
   // stochastic: add b * 2^8 + 2^7 to the FP32 bits, then truncate
   __device__ __forceinline__ static T enc_sr(float v, float s, uint32_t b) {
@@ -136,7 +144,15 @@ struct StepArgs {
   void* dst;  // pull: destination buffer; in-place: same as src
   float inv_tau, omt;
   float scale, inv_scale;
+  float neg_zero;        // -0.0f: the addend of the one-instruction 16-bit decode (dec_fast)
   uint32_t state;        // number of the state this launch produces (stochastic rounding key)
+  // fused dump (the DUMP step kernels): the full flag word for the DUMP bits
+  // and the rho/u buffers it writes (the same buffers as the EQU inputs above)
+  const uint32_t* flag;
+  float* mrho;
+  float* mvx;
+  float* mvy;
+  float* mvz;
 };
 
 }  // namespace culbm

@@ -259,6 +259,16 @@ class Simulator {
                       read_file(o_.init_folder + "/" + names[k] + ".dat", fields_[k]);
       if (!from_file_[k]) fields_[k].clear();
     }
+    // the plain pull run covers the domain with blockDim * gridDim exactly
+    // (staticPullRun, simulator_expt_platform.cu:962-967), checked after the
+    // init files as there; the library picks its own launch shape, so only the
+    // drop-in CLI keeps this rule
+    if (o_.cfg.stream_policy == CULBM_STREAM_PULL && o_.cfg.opt_policy == CULBM_OPT_NONE) {
+      const int32_t dom[3] = {o_.cfg.nx, o_.cfg.ny, o_.cfg.nz};
+      for (int k = 0; k < 3; ++k)
+        if (int64_t(o_.cfg.block_dim[k]) * o_.cfg.grid_dim[k] != dom[k])
+          throw std::runtime_error("Dimension of domain doesn't match computation grid!");
+    }
   }
 
   void init() {
@@ -299,9 +309,13 @@ class Simulator {
   }
   ~Simulator() { culbm_destroy(ctx_); }
 
+  // The reference's window loop (simulator_expt_platform.cu:963-991): each
+  // window's dump is armed as a deferred dump and written by the first step of
+  // the next window (no separate pass over the populations), its files once
+  // that window has run; the final window's dump runs as its own pass.
   void run() {
     const double cells = double(o_.cfg.nx) * o_.cfg.ny * o_.cfg.nz;
-    int64_t step = 0;
+    int64_t step = 0, pending = -1;
     int window = 0;
     while (step < o_.nstep) {
       int32_t done = 0;
@@ -313,10 +327,17 @@ class Simulator {
         std::printf("{\"window\": %d, \"steps\": %d, \"ms\": %.6f, \"mlups\": %.4f}\n", window, done,
                     double(ms), mlups);
       std::fflush(stdout);
+      if (pending >= 0) dump_res(pending);  // the window's first step wrote it
       step += done;
       ++window;
-      check(culbm_dump(ctx_), ctx_);
-      dump_res(step);
+      if (step < o_.nstep && done > 0) {
+        check(culbm_dump_deferred(ctx_), ctx_);
+        pending = step;
+      } else {
+        check(culbm_dump(ctx_), ctx_);
+        dump_res(step);
+        pending = -1;
+      }
       if (done == 0) break;  // dstep 0 makes no progress, as the reference's loop would spin
     }
   }

@@ -164,6 +164,11 @@ class Lattice:
     def dump(self) -> None:
         self._check(_lib.lib().culbm_dump(self._ctx))
 
+    def dump_deferred(self) -> None:
+        """Arm the dump of the current state; the next step writes it (fused),
+        or the next macros()/dump()/set_state()/... runs it first."""
+        self._check(_lib.lib().culbm_dump_deferred(self._ctx))
+
     def macros(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
         out = [np.empty(self.shape, dtype=np.float32) for _ in range(4)]
         self._check(_lib.lib().culbm_get_macros(self._ctx, *[_lib.ptr(a) for a in out]))

@@ -152,6 +152,19 @@ int culbm_profile_steps(culbm_ctx* ctx, int32_t nsteps, float* kernel_ms, int32_
  * rho/vx/vy/vz buffers where the DUMP bits are set. */
 int culbm_dump(culbm_ctx* ctx);
 
+/* The same dump, fused into the next step instead of a pass of its own (the
+ * reference's separate dump pass, kernel.cu:2634-2735 with dumpRes,
+ * simulator_expt_platform.cu:572-606, made to ride on the step that follows a
+ * window): the next step launch(es) gather exactly the populations the dump
+ * reads and write the same rho/u values.  The dump is delivered by whichever
+ * comes first: the next culbm_step / culbm_run_window / culbm_time_steps (its
+ * first step; the mixed policy runs it as a separate pass there), or a call
+ * that reads or replaces the fields or populations (culbm_get_macros,
+ * culbm_dump, culbm_set_state, culbm_init_ddf, culbm_set_ddf, culbm_adjoint),
+ * which runs it as a separate pass first.  A dumped field is that cell's EQU
+ * input afterwards, as after culbm_dump. */
+int culbm_dump_deferred(culbm_ctx* ctx);
+
 /* Copy the device rho/vx/vy/vz buffers to host arrays (NULL skips a field). */
 int culbm_get_macros(culbm_ctx* ctx, float* rho, float* vx, float* vy, float* vz);
 

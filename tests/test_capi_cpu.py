@@ -135,6 +135,12 @@ class TestCli:
         assert r.returncode == 1
         assert "--useRingDDFBuf is unimplemented unless --streamPolicy 1 --optPolicy 3." in r.stderr
 
+    def test_plain_grid_must_cover_the_domain(self):
+        r = run_cli("--domainDim", "[64,48,40]", "--streamPolicy", "0", "--optPolicy", "0")
+        assert r.returncode == 1 and "Dimension of domain doesn't match computation grid!" in r.stderr
+        r = run_cli("--domainDim", "[64,48,40]", "--gridDim", "[2,3,20]", "--dryRun")
+        assert r.returncode == 0, r.stderr
+
     def test_missing_flag_file(self, tmp_path):
         r = run_cli("--initStateFolder", str(tmp_path), "--domainDim", "[8,8,8]")
         assert r.returncode == 1 and f"File: {tmp_path}/flag.dat doesn't exist!" in r.stderr

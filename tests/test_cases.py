@@ -79,3 +79,36 @@ def test_state_roundtrip_and_defaults(tmp_path):
     (tmp_path / "a" / "flag.dat").unlink()
     with pytest.raises(FileNotFoundError, match="flag.dat doesn't exist"):
         cases.read_state(tmp_path / "a", st.domain)
+
+
+def test_correctness_case_follows_the_reference_generator_order():
+    """cases.config_correctness against a per-cell restatement of the
+    reference generator's assignment order (analysis/correctness/
+    generate.py:30-42): FLUID everywhere, then EQU|STORE on both x faces, then
+    the WALL cube, then WALL on both y faces, then WALL on both z faces; vx =
+    0.05 on x = 0, then 0 on the y and z faces.  Small shape, same rules."""
+    nx, ny, nz, (a, b) = 24, 12, 10, (4, 8)
+    st = cases.config_correctness(nx, ny, nz, cube=(a, b))
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                f = cases.FLUID
+                if x in (0, nx - 1):
+                    f = cases.EQUILIBRIUM
+                if a <= x < b and a <= y < b and a <= z < b:
+                    f = cases.WALL
+                if y in (0, ny - 1) or z in (0, nz - 1):
+                    f = cases.WALL
+                v = 0.05 if x == 0 and y not in (0, ny - 1) and z not in (0, nz - 1) else 0.0
+                assert st.flag[z, y, x] == f, (x, y, z)
+                assert st.vx[z, y, x] == np.float32(v), (x, y, z)
+    assert np.all(st.rho == 1.0) and not st.vy.any() and not st.vz.any()
+
+
+def test_write_state_subset_of_fields(tmp_path):
+    st = cases.config_correctness(16, 8, 8, cube=(2, 4))
+    cases.write_state(tmp_path, st, fields=("vx",))
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["flag.dat", "vx.dat"]
+    back = cases.read_state(tmp_path, st.domain)
+    assert np.array_equal(back.flag, st.flag) and np.array_equal(back.vx, st.vx)
+    assert np.all(back.rho == 1.0)

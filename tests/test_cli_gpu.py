@@ -19,6 +19,7 @@ ROOT = Path(__file__).resolve().parents[1]
 CLI = ROOT / "culbm_amd" / "bin" / "single_dev_expt_main"
 N = 32
 FIELDS = ("rho", "vx", "vy", "vz")
+PLAIN_GRID = ("--blockDim", "[32,16,2]", "--gridDim", f"[{N // 32},{N // 16},{N // 2}]")  # covers the domain
 
 
 def run_cli(init, out, *args):
@@ -42,7 +43,8 @@ def state(tmp_path_factory):
 def test_windows_and_dumps_match_the_oracle(state, tmp_path):
     from oracle import pyoracle as po
     st, folder = state
-    stdout, speeds = run_cli(folder, tmp_path, "--streamPolicy", 0, "--optPolicy", 0, "--nstep", 40, "--dstep", 20)
+    stdout, speeds = run_cli(folder, tmp_path, "--streamPolicy", 0, "--optPolicy", 0, "--nstep", 40, "--dstep", 20,
+                             *PLAIN_GRID)
     assert len(speeds) == 2 and all(s > 0 for s in speeds)
     assert "Init flag successfully, from file:" in stdout
     sim = po.Sim(st, 0.5, np.float32)  # the reference starts every run from rest
@@ -66,7 +68,7 @@ def test_blocked_in_place_windows_overshoot_and_match_plain_pull(state, tmp_path
     # windows advance in whole passes of innerLoop steps: 12, 24, 36, 48
     assert len(speeds) == 4
     assert sorted(p.name for p in out_b.glob("rho_*.dat")) == [f"rho_{s}.dat" for s in (12, 24, 36, 48)]
-    run_cli(folder, out_p, "--streamPolicy", 0, "--optPolicy", 0, "--nstep", 48, "--dstep", 48)
+    run_cli(folder, out_p, "--streamPolicy", 0, "--optPolicy", 0, "--nstep", 48, "--dstep", 48, *PLAIN_GRID)
     for name in FIELDS:
         a = (out_b / f"{name}_48.dat").read_bytes()
         b = (out_p / f"{name}_48.dat").read_bytes()
