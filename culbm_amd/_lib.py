@@ -115,6 +115,7 @@ SIGNATURES = {
     "culbm_set_ddf": (ctypes.c_int, [_VP, _F32]),
     "culbm_get_info": (ctypes.c_int, [_VP, ctypes.POINTER(Info)]),
     "culbm_moment_sums": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "culbm_moment_sums_noalloc": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "culbm_destroy": (ctypes.c_int, [_VP]),
     "culbm_last_error": (ctypes.c_char_p, [_VP]),
     "culbm_error_string": (ctypes.c_char_p, [ctypes.c_int]),

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -683,6 +684,73 @@ int32_t auto_depth(const culbm_ctx* ctx) {
   return std::max(2, std::min(d, nz));
 }
 
+// Moment sums: at most kSumParts workgroups, each writing one partial of the
+// four sums, then the four totals after them: the work array holds
+// 4 * kSumParts + 4 doubles whatever the domain.
+constexpr uint32_t kSumParts = 1024;
+
+constexpr size_t moment_sums_work_bytes() { return (4 * size_t(kSumParts) + 4) * sizeof(double); }
+
+// Launch the partial sums and their fixed-order total into `work` (device
+// memory of moment_sums_work_bytes()).  Every partial the total reads is
+// written by the same call, so `work` needs no clearing between calls.
+int moment_sums_launch(culbm_ctx* ctx, uint32_t flag_mask, double* work) {
+  StepArgs a = full_args(ctx);
+  const uint32_t nparts = std::min(a.nblk, kSumParts);
+  if (nparts == 0 || nparts > kSumParts) return fail(ctx, CULBM_ERR_INVALID, "bad partial-sum count %u", nparts);
+  const void* buf = ctx->inplace ? ctx->buf[0] : ctx->buf[ctx->cur];
+  const int rc = by_storage(ctx->cfg.storage, [&](auto s) {
+    using S = decltype(s);
+    if (!ctx->inplace)
+      hipLaunchKernelGGL((k_moment_sums<S, 0>), dim3(nparts), dim3(kBlock), 0, ctx->stream, a, buf, ctx->flag,
+                         flag_mask, work);
+    else if (ctx->parity == 1)  // last stores used the E rules
+      hipLaunchKernelGGL((k_moment_sums<S, 2>), dim3(nparts), dim3(kBlock), 0, ctx->stream, a, buf, ctx->flag,
+                         flag_mask, work);
+    else
+      hipLaunchKernelGGL((k_moment_sums<S, 1>), dim3(nparts), dim3(kBlock), 0, ctx->stream, a, buf, ctx->flag,
+                         flag_mask, work);
+    return hipGetLastError() == hipSuccess ? CULBM_OK : CULBM_ERR_HIP;
+  });
+  if (rc) return fail(ctx, rc, "moment-sum kernel launch failed");
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, ctx->stream, static_cast<const double*>(work), nparts,
+                     work + 4 * kSumParts);
+  HIPCHK(ctx, hipGetLastError());
+  return CULBM_OK;
+}
+
+}  // namespace
+
+// Work array of culbm_moment_sums_noalloc: the partial sums and the totals,
+// 4 * kSumParts + 4 doubles (32.8 KB), one copy on each device the library
+// runs on, held until the process ends.  In host code this name stands for the
+// symbol only; kernels and copies get the device address from scratch_on().
+static __device__ __align__(256) double scratch[4 * kSumParts + 4];
+static_assert(sizeof(scratch) == moment_sums_work_bytes(), "scratch holds the moment-sum work array");
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch copy, looked up once per
+// device (the caller has made ctx's device current).
+int scratch_on(culbm_ctx* ctx, double** out) {
+  const int dev = ctx->cfg.device;
+  if (dev < 0 || dev >= kMaxDevices) return fail(ctx, CULBM_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, CULBM_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return CULBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1023,33 +1091,28 @@ int culbm_moment_sums(culbm_ctx* ctx, uint32_t flag_mask, double* sums) {
   if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
   if (!sums) return fail(ctx, CULBM_ERR_INVALID, "sums is NULL");
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
-  StepArgs a = full_args(ctx);
-  double* part = nullptr;
-  HIPCHK(ctx, hipMalloc(&part, size_t(a.nblk) * 4 * sizeof(double)));
-  const void* buf = ctx->inplace ? ctx->buf[0] : ctx->buf[ctx->cur];
-  int rc = by_storage(ctx->cfg.storage, [&](auto s) {
-    using S = decltype(s);
-    if (!ctx->inplace)
-      hipLaunchKernelGGL((k_moment_sums<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
-                         ctx->flag, flag_mask, part);
-    else if (ctx->parity == 1)  // last stores used the E rules
-      hipLaunchKernelGGL((k_moment_sums<S, 2>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
-                         ctx->flag, flag_mask, part);
-    else
-      hipLaunchKernelGGL((k_moment_sums<S, 1>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, buf,
-                         ctx->flag, flag_mask, part);
-    return hipGetLastError() == hipSuccess ? CULBM_OK : CULBM_ERR_HIP;
-  });
-  std::vector<double> h(size_t(a.nblk) * 4);
-  if (!rc && hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+  double* work = nullptr;
+  HIPCHK(ctx, hipMalloc(&work, moment_sums_work_bytes()));
+  int rc = moment_sums_launch(ctx, flag_mask, work);
+  if (!rc && hipMemcpyAsync(sums, work + 4 * kSumParts, 4 * sizeof(double), hipMemcpyDeviceToHost,
                             ctx->stream) != hipSuccess)
-    rc = CULBM_ERR_HIP;
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(part);
-  if (rc) return fail(ctx, rc, "moment-sum kernel failed");
-  for (int q = 0; q < 4; ++q) sums[q] = 0.0;
-  for (uint32_t b = 0; b < a.nblk; ++b)
-    for (int q = 0; q < 4; ++q) sums[q] += h[4 * b + q];
+    rc = fail(ctx, CULBM_ERR_HIP, "device-to-host copy failed");
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = fail(ctx, CULBM_ERR_HIP, "moment-sum kernels failed");
+  (void)hipFree(work);
+  return rc;
+}
+
+int culbm_moment_sums_noalloc(culbm_ctx* ctx, uint32_t flag_mask, double* sums) {
+  if (!ctx) return fail(nullptr, CULBM_ERR_INVALID, "ctx is NULL");
+  if (!sums) return fail(ctx, CULBM_ERR_INVALID, "sums is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  double* work = nullptr;
+  if (int rc = scratch_on(ctx, &work)) return rc;
+  // the partial count is capped at kSumParts for every domain, so the work
+  // always fits the scratch array (moment_sums_launch rejects anything else)
+  if (int rc = moment_sums_launch(ctx, flag_mask, work)) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(sums, work + 4 * kSumParts, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CULBM_OK;
 }
 

@@ -802,30 +802,34 @@ __device__ __forceinline__ void stored_cell(const StepArgs& a, const Cell& k, co
 }
 
 // FP64 sums of rho = 1 + sum f and of the momentum sum c f over the cells whose
-// flag shares a bit with `mask` (every cell for mask 0); one partial sum of
-// each per workgroup, added in a fixed order by the host.
+// flag shares a bit with `mask` (every cell for mask 0).  A bounded grid walks
+// the cell blocks (workgroup g takes blocks g, g + G, ...), so the partial sums
+// number gridDim.x whatever the domain; k_sum_parts then adds them in a fixed
+// order.  One FP64 partial of each sum per workgroup.
 template <class S, int MODE>
 __global__ __launch_bounds__(kBlock) void k_moment_sums(const StepArgs a, const void* buf,
                                                         const uint32_t* flag, uint32_t mask,
                                                         double* part) {
-  const Cell k = locate(a, blockIdx.x);
   double m[4] = {0.0, 0.0, 0.0, 0.0};
-  if (k.ok && (mask == 0u || (flag[k.c] & mask))) {
-    float f[27];
-    stored_cell<S, MODE>(a, k, buf, f);
-    double rho = 1.0, jx = 0.0, jy = 0.0, jz = 0.0;
+  for (uint32_t blk = blockIdx.x; blk < a.nblk; blk += gridDim.x) {
+    const Cell k = locate(a, blk);
+    if (k.ok && (mask == 0u || (flag[k.c] & mask))) {
+      float f[27];
+      stored_cell<S, MODE>(a, k, buf, f);
+      double rho = 1.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-      const double v = f[i];
-      rho += v;
-      jx += cx_of(i) * v;
-      jy += cy_of(i) * v;
-      jz += cz_of(i) * v;
+      for (int i = 0; i < 27; ++i) {
+        const double v = f[i];
+        rho += v;
+        jx += cx_of(i) * v;
+        jy += cy_of(i) * v;
+        jz += cz_of(i) * v;
+      }
+      m[0] += rho;
+      m[1] += jx;
+      m[2] += jy;
+      m[3] += jz;
     }
-    m[0] = rho;
-    m[1] = jx;
-    m[2] = jy;
-    m[3] = jz;
   }
   __shared__ double red[4][kBlock / 64];
 #pragma unroll
@@ -841,6 +845,15 @@ __global__ __launch_bounds__(kBlock) void k_moment_sums(const StepArgs a, const 
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
     part[4 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// The four totals of `nparts` partial sums, added in index order.
+__global__ __launch_bounds__(64) void k_sum_parts(const double* part, uint32_t nparts, double* out) {
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (uint32_t b = 0; b < nparts; ++b) s += part[4 * b + threadIdx.x];
+    out[threadIdx.x] = s;
   }
 }
 

@@ -182,6 +182,16 @@ int culbm_set_ddf(culbm_ctx* ctx, const float* ddf);
  * cell for 0).  The mass-conservation check; no reference counterpart. */
 int culbm_moment_sums(culbm_ctx* ctx, uint32_t flag_mask, double* sums);
 
+/* culbm_moment_sums without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels, and so the same bits, as culbm_moment_sums; their partial sums live
+ * in a fixed 32.8 KB array of the library (4 * 1024 + 4 doubles, one copy per
+ * device, held until the process ends), whatever the domain size.  Calls that
+ * overlap on one device share that array and race: issue them one at a time
+ * per device. */
+int culbm_moment_sums_noalloc(culbm_ctx* ctx, uint32_t flag_mask, double* sums);
+
 int culbm_get_info(culbm_ctx* ctx, culbm_info* info);
 
 /* Release device memory and the stream.  NULL is accepted. */
