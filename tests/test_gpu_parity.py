@@ -369,3 +369,86 @@ def test_ragged_domains_all_policies_match_the_oracle(dom):
         lat.step(steps)
         assert_bitexact(lat.get_ddf(), want, f"{dom} ({sp},{op}) I={il} D={wd}")
         lat.close()
+
+
+# ---- 16-bit storage: every policy against the host restatement ---------------------
+
+@pytest.mark.parametrize("storage,rounding", [("fp16", "nearest"), ("bf16", "nearest"), ("fp16", "stochastic")])
+@pytest.mark.parametrize("outlet", ["zero_gradient", "equilibrium"])
+def test_16bit_variants_bitexact_vs_oracle(storage, rounding, outlet):
+    """Every policy (pull, in place, wavefront, static, mixed) in FP16 / BF16,
+    nearest and stochastic, against the 16-bit host restatement, bit for bit."""
+    st = obstacle_case(nx=38, ny=29, nz=31, outlet=outlet)
+    steps = 17
+    fmt = storage
+    lat0 = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding)
+    scale = lat0.info()["storage_scale"]
+    lat0.close()
+    o = pyoracle.Sim(st, 0.7, fmt, scale=scale, rounding=rounding)
+    o.init_feq()
+    o.step(steps)
+    want_f = o.ddf()
+    want_m = o.dump()
+    for sp, op, il, wd in [(0, 0, 5, 0), (1, 0, 5, 0), (0, 3, 5, 3), (1, 3, 5, 3), (1, 3, 1, 2), (1, 3, 4, 0),
+                           (0, 1, 3, 4), (1, 1, 5, 2), (0, 2, 5, 0), (0, 2, 3, 7)]:
+        lat = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding, stream_policy=sp,
+                   opt_policy=op, inner_loop=il, wave_depth=wd)
+        lat.step(steps)
+        lat.dump()
+        what = f"({sp},{op}) I={il} D={wd} {storage} {rounding} {outlet}"
+        assert_bitexact(lat.get_ddf(), want_f, what + " populations")
+        for a, b in zip(lat.macros(), want_m):
+            assert_bitexact(a, b, what + " macros")
+        lat.close()
+
+
+RAGGED16 = [(2, 1, 1), (2, 3, 2), (4, 1, 5), (64, 2, 3), (66, 3, 2), (130, 2, 2), (8, 9, 17)]
+
+
+@pytest.mark.parametrize("dom", RAGGED16)
+def test_16bit_ragged_domains_match_the_oracle(dom):
+    """Every cell FLUID (every face clamps), FP16 from a random near-equilibrium
+    start, on domains down to nx = 2: each policy equals the FP16 host
+    restatement bit for bit."""
+    nx, ny, nz = dom
+    rng = np.random.default_rng(nx * 10007 + ny * 101 + nz)
+    flag = np.full((nz, ny, nx), cases.FLUID, np.uint32)
+    rho = (1.0 + rng.uniform(-1e-3, 1e-3, (nz, ny, nx))).astype(np.float32)
+    u = [rng.uniform(-0.02, 0.02, (nz, ny, nx)).astype(np.float32) for _ in range(3)]
+    state = cases.State(flag, rho, *u)
+    steps = 7
+    o = pyoracle.Sim(state, 0.8, "fp16", scale=4096.0)
+    o.init_feq()
+    o.step(steps)
+    want = o.ddf()
+    for sp, op, il, wd in [(0, 0, 5, 0), (1, 0, 5, 0), (0, 3, 3, 0), (1, 3, 3, 1), (1, 3, 5, 0),
+                           (0, 1, 2, 0), (1, 1, 3, 2), (0, 2, 3, 0)]:
+        lat = Lattice(LatticeConfig(domain=dom, inv_tau=0.8, stream_policy=sp, opt_policy=op,
+                                    inner_loop=il, wave_depth=wd, storage="fp16"))
+        lat.set_state(flag, rho, *u)
+        lat.init_ddf("feq")
+        lat.step(steps)
+        assert_bitexact(lat.get_ddf(), want, f"{dom} ({sp},{op}) I={il} D={wd} fp16")
+        lat.close()
+
+
+@pytest.mark.parametrize("sp,op", POLICIES)
+def test_16bit_stream_index_tags(sp, op):
+    """Cell-id tags through pure streaming in FP16 with scale 1 (every tag an
+    exact binary16 integer): the 16-bit addressing, face clamp and ghost copies
+    against the clamped-streaming expectation."""
+    nx, ny, nz = 6, 3, 4
+    st = cases.channel(nx, ny, nz)
+    st.flag[...] = cases.LOAD | cases.STORE
+    lat = Lattice(LatticeConfig(domain=(nx, ny, nz), stream_policy=sp, opt_policy=op, inner_loop=3,
+                                wave_depth=2, storage="fp16", storage_scale=1.0))
+    lat.set_state(st.flag)
+    n = nx * ny * nz
+    tags = (np.arange(27)[:, None] * n + np.arange(n)[None, :]).astype(np.float32).reshape(27, nz, ny, nx)
+    assert tags.max() < 2048
+    for steps in (1, 4):
+        lat.set_ddf(tags)
+        lat.step(steps)
+        got = lat.get_ddf()
+        want = stream_expect(tags, steps)
+        assert np.array_equal(got, want), f"policy ({sp},{op}) after {steps} steps"
