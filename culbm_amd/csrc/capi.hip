@@ -41,6 +41,7 @@ struct culbm_ctx {
   float* vz = nullptr;
   int64_t step = 0;
   float scale = 1.0f, inv_scale = 1.0f;
+  double state_bound = 0.0;  // feq_bound over the cells of the last culbm_set_state
   int32_t wave_depth = 0;
   int32_t tiles_x = 0, tiles_y = 0, nchunk = 0;  // mixed policy launch shape
   // static policy: one instantiated graph per starting state of a full pass
@@ -91,10 +92,51 @@ bool is_pow2_scale(float s) {
 
 bool blocked(const culbm_config& c) { return c.opt_policy != CULBM_OPT_NONE; }
 
-// Default scale of the 16-bit formats.  Shifted populations stay well below
-// 1 in magnitude, so FP16 takes 2^12 to move them away from its subnormal
-// range; BF16 has FP32's exponent range and needs none.
-float default_scale(int32_t storage) { return storage == CULBM_STORAGE_FP16 ? 4096.0f : 1.0f; }
+// Scale of the 16-bit formats, derived per run (storage_scale 0).  FP16 keeps
+// (f_i - w_i) * s in binary16, finite up to 65504 and normal from 2^-14: the
+// scale is the largest power of two that keeps kScaleHeadroom times the run's
+// bound on |f_i - w_i| finite, so that the small deviations stay as far from
+// the subnormals as the range allows.  A bound below kBoundFloor (a state at
+// rest) counts as kBoundFloor, which leaves room for a flow of |u| ~ 0.1 that
+// a later culbm_set_state may bring without re-initialising the populations.
+// BF16 has FP32's exponent range and takes 1.
+constexpr double kScaleHeadroom = 16.0;
+constexpr double kBoundFloor = 1.0 / 32.0;
+constexpr int kMaxScaleExp = 20;
+
+float derived_scale(int32_t storage, double bound) {
+  if (storage != CULBM_STORAGE_FP16) return 1.0f;
+  const double q = 65504.0 / (kScaleHeadroom * std::max(bound, kBoundFloor));
+  int e;
+  (void)std::frexp(q, &e);  // q = m 2^e with m in [0.5, 1): the largest power of two <= q is 2^(e-1)
+  return std::ldexp(1.0f, std::min(std::max(e - 1, 0), kMaxScaleExp));
+}
+
+// Bound on |feq_i(rho, u) - w_i| over the 27 directions of a cell:
+// w_i (|rho - 1| + rho (3 |c.u| + 4.5 (c.u)^2 + 1.5 u^2)) with w_i <= 8/27 and
+// |c.u| <= |ux| + |uy| + |uz|.
+double feq_bound(float rho, float ux, float uy, float uz) {
+  const double r = rho, s = std::fabs(double(ux)) + std::fabs(double(uy)) + std::fabs(double(uz));
+  const double u2 = double(ux) * ux + double(uy) * uy + double(uz) * uz;
+  return (8.0 / 27.0) * (std::fabs(r - 1.0) + std::fabs(r) * (3.0 * s + 4.5 * s * s + 1.5 * u2));
+}
+
+bool auto_scale(const culbm_ctx* ctx) {
+  return ctx->cfg.storage != CULBM_STORAGE_FP32 && ctx->cfg.storage_scale == 0.0f;
+}
+
+void set_scale(culbm_ctx* ctx, float s) {
+  if (s == ctx->scale) return;
+  ctx->scale = s;
+  ctx->inv_scale = 1.0f / s;
+  // captured passes carry the old scale in their kernel arguments
+  (void)hipStreamSynchronize(ctx->stream);
+  for (hipGraphExec_t& g : ctx->pass_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
 
 // Population layout of a configuration: pitches and direction stride.  The one
 // place that defines it, so the 32-bit offset check of culbm_validate tests
@@ -801,7 +843,7 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
   ctx->sr = c.rounding == CULBM_ROUND_STOCHASTIC;
   ctx->bpp = bytes_per_pop(c.storage);
   ctx->scale = c.storage == CULBM_STORAGE_FP32 ? 1.0f
-               : (c.storage_scale != 0.0f ? c.storage_scale : default_scale(c.storage));
+               : (c.storage_scale != 0.0f ? c.storage_scale : derived_scale(c.storage, 0.0));
   ctx->inv_scale = 1.0f / ctx->scale;
   {
     const Layout L = layout_of(c);
@@ -917,6 +959,12 @@ int culbm_set_state(culbm_ctx* ctx, const uint32_t* flag, const float* rho, cons
   const float* src[4] = {rho, vx, vy, vz};
   float* dst[4] = {ctx->rho, ctx->vx, ctx->vy, ctx->vz};
   const float dflt[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+  if (auto_scale(ctx)) {
+    double b = 0.0;
+    for (size_t i = 0; i < nb; ++i)
+      b = std::max(b, feq_bound(rho ? rho[i] : 1.0f, vx ? vx[i] : 0.0f, vy ? vy[i] : 0.0f, vz ? vz[i] : 0.0f));
+    ctx->state_bound = b;
+  }
   std::vector<float> fill;
   for (int k = 0; k < 4; ++k) {
     if (src[k]) {
@@ -937,6 +985,8 @@ int culbm_init_ddf(culbm_ctx* ctx, int32_t mode) {
     return fail(ctx, CULBM_ERR_INVALID, "init mode must be 0 (rest) or 1 (feq), got %d", mode);
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
   if (int rc = flush_dump(ctx)) return rc;
+  // every population is encoded anew: the point where a derived scale is set
+  if (auto_scale(ctx)) set_scale(ctx, derived_scale(ctx->cfg.storage, ctx->state_bound));
   for (int b = 0; b < 2; ++b)
     if (ctx->buf[b]) {
       const int rc = zero_device(ctx, ctx->buf[b], ddf_bytes(ctx));
@@ -1072,6 +1122,13 @@ int culbm_set_ddf(culbm_ctx* ctx, const float* ddf) {
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
   if (int rc = flush_dump(ctx)) return rc;
   const size_t bytes = size_t(27) * size_t(ctx->n) * 4;
+  if (auto_scale(ctx)) {
+    // the populations given (shifted, f_i - w_i) and the state's equilibria both have to fit
+    float m = 0.0f;
+    const size_t total = size_t(27) * size_t(ctx->n);
+    for (size_t c = 0; c < total; ++c) m = std::max(m, std::fabs(ddf[c]));
+    set_scale(ctx, derived_scale(ctx->cfg.storage, std::max(ctx->state_bound, double(m))));
+  }
   float* plain = nullptr;
   HIPCHK(ctx, hipMalloc(&plain, bytes));
   int rc = CULBM_OK;

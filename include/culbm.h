@@ -90,7 +90,7 @@ typedef struct culbm_config {
   int32_t use_ring;         /* --useRingDDFBuf                               */
   int32_t device;           /* --devId                                       */
   int32_t storage;          /* CULBM_STORAGE_*                               */
-  float storage_scale;      /* 16-bit scale (power of two); 0 = default      */
+  float storage_scale;      /* 16-bit scale (power of two); 0 = per run      */
   int32_t wave_depth;       /* planes per wavefront chunk; 0 = automatic     */
   int32_t rounding;         /* CULBM_ROUND_* of the 16-bit stores            */
   int32_t reserved[6];

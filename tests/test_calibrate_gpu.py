@@ -48,7 +48,7 @@ def test_population_pattern_is_faster_from_the_cache_than_from_hbm():
 def test_selector_times_every_candidate_and_returns_the_fastest():
     from culbm_amd.calibrate import select
 
-    res = select((64, 48, 40), stream_policy=1, storage="fp32", inner_loops=(3,), depths=(0, 4, 8), steps=6)
+    res = select((64, 48, 40), stream_policy=1, storage="fp32", inner_loops=(3,), depths=(0, 4, 8), steps=6, develop=10)
     rows = res["candidates"]
     assert [r["requested_depth"] for r in rows] == [0, 4, 8]
     assert all(r["mlups"] > 0 for r in rows)

@@ -236,15 +236,41 @@ def test_set_get_ddf_roundtrip(sp, op, storage):
     lat = make(st, stream_policy=sp, opt_policy=op, storage=storage)
     rng = np.random.default_rng(3)
     f = rng.normal(0, 1e-2, (27,) + st.shape).astype(np.float32)
-    if storage == "fp16":
+    lat.set_ddf(f)
+    if storage == "fp16":  # the scale set_ddf derived from f and the state
         scale = lat.info()["storage_scale"]
         enc = np.zeros(f.size, np.uint16)
         pyoracle.lib().orc_encode16(1, (f * np.float32(scale)).reshape(-1), enc, f.size)
         dec = np.zeros(f.size, np.float32)
         pyoracle.lib().orc_decode16(1, enc, dec, f.size)
         f = (dec * np.float32(1 / scale)).reshape(f.shape)
-    lat.set_ddf(f)
     assert_bitexact(lat.get_ddf(), f, "roundtrip")
+
+
+def test_16bit_scale_is_derived_per_run():
+    """FP16's scale is the largest power of two with 16 B s <= 65504, B the
+    bound on |f_i - w_i| of the run's state (at least 1/32) or of the
+    populations given to set_ddf; BF16 takes 1; an explicit scale is kept."""
+    st = obstacle_case(nx=19, ny=17, nz=15)  # rho 1 +- 2e-3, u_in 0.06: B = 0.0604
+
+    def scale_of(state, storage="fp16", **kw):
+        lat = make(state, storage=storage, **kw)
+        s = lat.info()["storage_scale"]
+        lat.close()
+        return s
+
+    assert scale_of(st) == 2.0 ** 16  # 65504 / (16 B) = 67.7k
+    assert scale_of(st, storage="bf16") == 1.0
+    assert scale_of(st, storage_scale=512.0) == 512.0
+    fast = cases.State(st.flag, st.rho, st.vx * 8, st.vy, st.vz)  # u_in 0.48: B = 0.84 -> 4.9k
+    assert scale_of(fast) == 2.0 ** 12
+    lat = make(st, storage="fp16")
+    f = np.zeros((27,) + st.shape, np.float32)
+    f[3, 2, 2, 2] = 2.0  # 65504 / 32 = 2047
+    lat.set_ddf(f)
+    assert lat.info()["storage_scale"] == 2.0 ** 10
+    assert lat.get_ddf()[3, 2, 2, 2] == 2.0
+    lat.close()
 
 
 # ---- the optimised workload at full size --------------------------------------
@@ -417,16 +443,18 @@ def test_16bit_ragged_domains_match_the_oracle(dom):
     u = [rng.uniform(-0.02, 0.02, (nz, ny, nx)).astype(np.float32) for _ in range(3)]
     state = cases.State(flag, rho, *u)
     steps = 7
-    o = pyoracle.Sim(state, 0.8, "fp16", scale=4096.0)
-    o.init_feq()
-    o.step(steps)
-    want = o.ddf()
+    want = None
     for sp, op, il, wd in [(0, 0, 5, 0), (1, 0, 5, 0), (0, 3, 3, 0), (1, 3, 3, 1), (1, 3, 5, 0),
                            (0, 1, 2, 0), (1, 1, 3, 2), (0, 2, 3, 0)]:
         lat = Lattice(LatticeConfig(domain=dom, inv_tau=0.8, stream_policy=sp, opt_policy=op,
                                     inner_loop=il, wave_depth=wd, storage="fp16"))
         lat.set_state(flag, rho, *u)
         lat.init_ddf("feq")
+        if want is None:  # the scale derived from this state
+            o = pyoracle.Sim(state, 0.8, "fp16", scale=lat.info()["storage_scale"])
+            o.init_feq()
+            o.step(steps)
+            want = o.ddf()
         lat.step(steps)
         assert_bitexact(lat.get_ddf(), want, f"{dom} ({sp},{op}) I={il} D={wd} fp16")
         lat.close()
