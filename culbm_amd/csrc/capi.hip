@@ -516,9 +516,9 @@ int fused_pass(culbm_ctx* ctx, int32_t T) {
     bool sr = false;
     if constexpr (S::kBytes == 2) sr = ctx->sr;
     if (sr)
-      hipLaunchKernelGGL((k_fused2<S, true>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
+      hipLaunchKernelGGL((k_fused2<S, true>), dim3(nblk), dim3(FusedShape<S::kBytes>::BLOCK), 0, ctx->stream, fa);
     else
-      hipLaunchKernelGGL((k_fused2<S, false>), dim3(nblk), dim3(kFBlock), 0, ctx->stream, fa);
+      hipLaunchKernelGGL((k_fused2<S, false>), dim3(nblk), dim3(FusedShape<S::kBytes>::BLOCK), 0, ctx->stream, fa);
     HIPCHK(ctx, hipGetLastError());
     if (int rc = prof_end(ctx)) return rc;
     ctx->cur ^= 1;
@@ -877,13 +877,15 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
     by_storage(c.storage, [&](auto s) {
       using S = decltype(s);
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_fused2<S, false>),
-                                                       kFBlock, 0) != hipSuccess)
+                                                       FusedShape<S::kBytes>::BLOCK, 0) != hipSuccess)
         per_cu = 1;
       return 0;
     });
     const int slots = ncu * std::max(per_cu, 1);
-    ctx->tiles_x = (c.nx + kFTX - 1) / kFTX;
-    ctx->tiles_y = (c.ny + kFTY - 1) / kFTY;
+    const int ftx = ctx->bpp == 4 ? FusedShape<4>::TX : FusedShape<2>::TX;
+    const int fty = ctx->bpp == 4 ? FusedShape<4>::TY : FusedShape<2>::TY;
+    ctx->tiles_x = (c.nx + ftx - 1) / ftx;
+    ctx->tiles_y = (c.ny + fty - 1) / fty;
     const double tiles = double(ctx->tiles_x) * ctx->tiles_y;
     if (c.wave_depth > 0) {
       ctx->wave_depth = std::min(c.wave_depth, c.nz);

@@ -464,9 +464,24 @@ __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
 // bit.  A cell without STORE keeps its populations in both buffers (see
 // culbm_set_state), so its intermediate value is its src value.
 
-constexpr int kFTX = 32, kFTY = 8;
-constexpr int kFAX = kFTX + 2, kFAY = kFTY + 2, kFA = kFAX * kFAY;  // 34 x 10 = 340 A cells
-constexpr int kFBlock = 384;  // six waves: the 340 A cells of a plane in one round
+// Tile of a workgroup per storage width: TX x TY B cells, (TX+2)(TY+2) A cells
+// (the tile and a one-cell xy halo), one thread per A cell.
+template <int BYTES>
+struct FusedTile;
+template <>
+struct FusedTile<4> {
+  static constexpr int TX = 32, TY = 16, BLOCK = 640;  // 612 A cells, ten waves
+};
+template <>
+struct FusedTile<2> {
+  static constexpr int TX = 96, TY = 8, BLOCK = 1024;  // 980 A cells, sixteen waves
+};
+template <int BYTES>
+struct FusedShape : FusedTile<BYTES> {
+  using Base = FusedTile<BYTES>;
+  static constexpr int AX = Base::TX + 2, AY = Base::TY + 2, A = AX * AY;
+  static_assert(A <= Base::BLOCK && Base::BLOCK % 64 == 0 && Base::BLOCK <= 1024, "one A cell per thread");
+};
 
 struct FusedArgs {
   StepArgs a;
@@ -481,11 +496,12 @@ struct FusedArgs {
 // live 3, 2 and 1 plane(s): 54 values per A cell instead of 81.
 template <class T>
 struct FusedRing {
-  T up[3][9][kFA];  // directions 18..26 (c_z = +1), slot = plane % 3
-  T mid[2][9][kFA]; // directions 9..17 (c_z = 0), slot = plane % 2
-  T dn[9][kFA];     // directions 0..8 (c_z = -1), the latest plane
-  uint8_t fl[2][kFA];  // step bits of the A cells, slot = plane % 2 (the B step's flags)
-  float eq[2][2][kFTY][4];  // EQU inputs of the B cells on the low / high x face, slot = plane % 2
+  using F = FusedShape<sizeof(T)>;
+  T up[3][9][F::A];  // directions 18..26 (c_z = +1), slot = plane % 3
+  T mid[2][9][F::A]; // directions 9..17 (c_z = 0), slot = plane % 2
+  T dn[9][F::A];     // directions 0..8 (c_z = -1), the latest plane
+  uint8_t fl[2][F::A];  // step bits of the A cells, slot = plane % 2 (the B step's flags)
+  float eq[2][2][F::TY][4];  // EQU inputs of the B cells on the low / high x face, slot = plane % 2
 };
 
 // Pulled (not yet updated) populations of one A cell, gathered a plane ahead
@@ -504,8 +520,9 @@ template <class S>
 // is still in flight, instead of for every outstanding load.
 __device__ __forceinline__ void fused_a_gather(const StepArgs& a, int32_t x0, int32_t y0, int ac,
                                                int32_t p, APending& q) {
-  const int32_t gx = x0 - 1 + ac % kFAX, gy = y0 - 1 + ac / kFAX;
-  q.ok = ac < kFA && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny;
+  using F = FusedShape<sizeof(typename S::T)>;
+  const int32_t gx = x0 - 1 + ac % F::AX, gy = y0 - 1 + ac / F::AX;
+  q.ok = ac < F::A && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny;
   Cell k;
   k.x = min(max(gx, 0), a.nx - 1);
   k.y = min(max(gy, 0), a.ny - 1);
@@ -557,8 +574,8 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
   const int su = p % 3, sm = p % 2;
   r.fl[sm][ac] = uint8_t(q.fl);
   // EQU inputs of the x-face cells among the B cells, for the second step
-  const int ly = ac / kFAX - 1;
-  if (qe.ok && ly >= 0 && ly < kFTY) {
+  const int ly = ac / FusedShape<sizeof(T)>::AX - 1;
+  if (qe.ok && ly >= 0 && ly < FusedShape<sizeof(T)>::TY) {
     float* d = r.eq[sm][gx == 0 ? 0 : 1][ly];
     d[0] = qe.rho;
     d[1] = qe.vx;
@@ -575,9 +592,10 @@ __device__ __forceinline__ void fused_a_finish(const StepArgs& a, int ac, int32_
 
 template <class S, bool SR>
 // Three waves per SIMD: two workgroups per CU, as many as the FP32 ring's LDS allows.
-__global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_fused2(
+__global__ __launch_bounds__(FusedShape<S::kBytes>::BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_fused2(
     const FusedArgs fa) {
   using T = typename S::T;
+  using F = FusedShape<sizeof(T)>;
   __shared__ FusedRing<T> r;
   const uint32_t ntile = uint32_t(fa.tiles_x) * fa.tiles_y * fa.nchunk;
   int32_t tx, ty, zc;
@@ -608,11 +626,11 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
     ty = int32_t((w / uint32_t(fa.tiles_x)) % uint32_t(fa.tiles_y));
     zc = int32_t(w / (uint32_t(fa.tiles_x) * fa.tiles_y));
   }
-  const int32_t x0 = tx * kFTX, y0 = ty * kFTY;
+  const int32_t x0 = tx * F::TX, y0 = ty * F::TY;
   const int32_t nx = fa.a.nx, ny = fa.a.ny, nz = fa.a.nz;
   const int32_t zs = zc * fa.zchunk, ze = min(nz, zs + fa.zchunk);
   const int t = threadIdx.x;
-  const bool a_ok = t < kFA;
+  const bool a_ok = t < F::A;
 
   // A planes gathered ahead of their first step: two for FP32 (the gather
   // latency then spans two planes' B steps), one for 16-bit storage
@@ -630,14 +648,14 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
   fused_a_gather<S>(fa.a, x0, y0, t, min(zs + 1, nz - 1), q0);
   if (kAhead == 2) fused_a_gather<S>(fa.a, x0, y0, t, min(zs + 2, nz - 1), q1);
 
-  const int bx = t % kFTX, by = t / kFTX;
+  const int bx = t % F::TX, by = t / F::TX;
   const int32_t gx = x0 + bx, gy = y0 + by;
-  const bool b_ok = t < kFTX * kFTY && gx < nx && gy < ny;
+  const bool b_ok = t < F::TX * F::TY && gx < nx && gy < ny;
   // local A coordinates of the (clamped) source columns
   const int lxm = gx == 0 ? bx + 1 : bx, lxp = gx == nx - 1 ? bx + 1 : bx + 2;
   const int lym = gy == 0 ? by + 1 : by, lyp = gy == ny - 1 ? by + 1 : by + 2;
   const uint32_t cxy = uint32_t(gy) * uint32_t(nx) + uint32_t(gx), pl = uint32_t(nx) * uint32_t(ny);
-  const int bc = (by + 1) * kFAX + bx + 1;  // the B cell among the A cells
+  const int bc = (by + 1) * F::AX + bx + 1;  // the B cell among the A cells
   // B plane z; q holds the gathered A plane z+1 and receives A plane z+1+kAhead.
   // The B step issues no global load: its flag byte comes from the A step's
   // copy in LDS, so no wait on a global load can hold back the A gathers
@@ -656,7 +674,7 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
         const int cx = cx_of(i), cy = cy_of(i), cz = cz_of(i);
         const int lx = cx == 1 ? lxm : (cx == -1 ? lxp : bx + 1);
         const int ly = cy == 1 ? lym : (cy == -1 ? lyp : by + 1);
-        const int a = ly * kFAX + lx;
+        const int a = ly * F::AX + lx;
         const T v = cz == 1 ? r.up[su][i - 18][a] : (cz == 0 ? r.mid[sm][i - 9][a] : r.dn[i][a]);
         f[i] = S::dec(v, fa.a.inv_scale);
       }
