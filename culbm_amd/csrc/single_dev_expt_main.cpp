@@ -62,7 +62,7 @@ const char* kUsage =
     "  --useRingDDFBuf           One global DDF buffer for Dynamic L2 in-place streaming.\n"
     "  --innerLoop INT           # of iterations in halo-blocking. [5]\n"
     "  --storage TEXT            Population storage {fp32,fp16,bf16} [fp32]\n"
-    "  --storageScale FLOAT      16-bit storage scale (power of two, 0 = default) [0]\n"
+    "  --storageScale FLOAT      16-bit storage scale (power of two, 0 = per run) [0]\n"
     "  --waveDepth INT           Planes per temporal-blocking chunk (0 = automatic) [0]\n"
     "  --initDDF TEXT            Populations at start {rest,feq} [rest]\n"
     "  --rounding TEXT           16-bit stores {nearest,stochastic} [nearest]\n"
