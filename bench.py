@@ -49,6 +49,7 @@ FORMS = {
     "fp16_mixed": (0, 2, "fp16", "FP16-shifted pull + mixed blocking (two steps per launch, LDS)"),
     "fp16_sr_inplace_blocked": (1, 3, "fp16", "FP16-shifted in-place + dynamic blocking, stochastic rounding",
                                 "stochastic"),
+    "fp16_sr_mixed": (0, 2, "fp16", "FP16-shifted pull + mixed blocking, stochastic rounding", "stochastic"),
 }
 HEADLINE = "fp32_inplace_blocked"
 
@@ -406,8 +407,10 @@ def main(argv=None) -> int:
     if tpath.exists():
         t = json.loads(tpath.read_text()).get(ns.form)
         if t:
-            cells_per_launch = DOMAIN[0] * DOMAIN[1] * DOMAIN[2] / head["launches_per_step"]
-            traffic = round(t.get("bytes_per_cell", t.get("bytes_per_update", 0.0)) * cells_per_launch)
+            ncell = DOMAIN[0] * DOMAIN[1] * DOMAIN[2]
+            # a k_fused2 launch makes two updates of every cell; a cell launch one update of its chunk
+            updates_per_launch = ncell / head["launches_per_step"] if single else 2 * ncell
+            traffic = round(t.get("bytes_per_cell", t.get("bytes_per_update", 0.0)) * updates_per_launch)
     line = {
         "metric": "MLUPS D3Q27 at 288x272x280 (% HBM roofline); adjoint/forward step ratio",
         "value": round(head["mlups"], 3),

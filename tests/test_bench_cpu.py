@@ -18,7 +18,10 @@ def test_traffic_entries_carry_bytes_per_cell_update():
     assert bench.HEADLINE in t
     for form, e in t.items():
         assert form in bench.FORMS, form
-        assert 100.0 < e["bytes_per_cell"] < 400.0, (form, e)
+        # per cell-update, against one step's algorithmic bytes (27 reads + 27 writes);
+        # a launch that makes two steps (mixed policy) can go down to half of them
+        alg = 54 * (4 if bench.FORMS[form][2] == "fp32" else 2)
+        assert 0.5 * alg <= e["bytes_per_cell"] < 2.0 * alg, (form, e)
 
 
 def test_more_than_one_gpu_is_refused_with_a_message():
