@@ -348,6 +348,27 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       hipLaunchKernelGGL((k_pull<S, false>), dim3(s.nblk), dim3(kBlock), 0, ctx->stream, s);
     ++launches;
   };
+  // f^{t_out} from f^{t_out - 2} in one launch (two steps through LDS, the same
+  // bits as two plain steps), where the state in between is not needed
+  auto step2 = [&](const void* src, void* dst, int32_t t_out) {
+    FusedArgs fa{};
+    fa.a = a;
+    fa.a.src = src;
+    fa.a.dst = dst;
+    fa.a.state = s0 + uint32_t(t_out) - 1u;  // the launch produces states state and state + 1
+    fa.tiles_x = ctx->tiles_x;
+    fa.tiles_y = ctx->tiles_y;
+    fa.zchunk = ctx->fzchunk;
+    fa.nchunk = ctx->nchunk;
+    const uint32_t nblk = uint32_t(ctx->tiles_x) * ctx->tiles_y * ctx->nchunk;
+    bool sr = false;
+    if constexpr (S::kBytes == 2) sr = ctx->sr;
+    if (sr)
+      hipLaunchKernelGGL((k_fused2<S, true>), dim3(nblk), dim3(FusedShape<S::kBytes>::BLOCK), 0, ctx->stream, fa);
+    else
+      hipLaunchKernelGGL((k_fused2<S, false>), dim3(nblk), dim3(FusedShape<S::kBytes>::BLOCK), 0, ctx->stream, fa);
+    ++launches;
+  };
   auto in_window = [&](int32_t t) { return t >= sp.t_begin && t <= sp.t_end; };
   // the first sweep accumulates J; the reverse sweep adds the objective's gradient to mu
   auto objective = [&](const void* f, float* m) {
@@ -371,16 +392,19 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   if ((rc = timer.mark(0))) return rc;
   const void* cur = slot[0];
   size_t next_store = 0;
-  for (int32_t t = 0;; ++t) {
+  auto stored_at = [&](int32_t t) { return next_store < plan.sweep.size() && plan.sweep[next_store].first == t; };
+  for (int32_t t = 0;;) {
     if (in_window(t)) objective(cur, nullptr);
     if (t == N) break;
-    void* dst;
-    if (next_store < plan.sweep.size() && plan.sweep[next_store].first == t + 1)
-      dst = slot[size_t(plan.sweep[next_store++].second)];
+    // two steps in one launch when f^{t+1} is neither evaluated nor kept
+    const int32_t n = (t + 2 <= N && !in_window(t + 1) && !stored_at(t + 1)) ? 2 : 1;
+    void* dst = stored_at(t + n) ? slot[size_t(plan.sweep[next_store++].second)] : free_work(cur);
+    if (n == 2)
+      step2(cur, dst, t + 2);
     else
-      dst = free_work(cur);
-    step(cur, dst, t + 1);
+      step(cur, dst, t + 1);
     cur = dst;
+    t += n;
   }
   if ((rc = timer.mark(0))) return rc;
   HIPCHK(ctx, hipGetLastError());
@@ -410,10 +434,15 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
     if (o.kind == 0) {
       if ((rc = timer.mark(1))) return rc;
       const void* p = src;
-      for (int32_t s = 0; s < o.steps; ++s) {
-        void* dst = (s == o.steps - 1 && o.dst >= 0) ? slot[size_t(o.dst)] : free_work(p);
-        step(p, dst, o.t + s + 1);
+      for (int32_t s = 0; s < o.steps;) {
+        const int32_t n = s + 2 <= o.steps ? 2 : 1;  // pairs of steps in one launch
+        void* dst = (s + n == o.steps && o.dst >= 0) ? slot[size_t(o.dst)] : free_work(p);
+        if (n == 2)
+          step2(p, dst, o.t + s + 2);
+        else
+          step(p, dst, o.t + s + 1);
         p = dst;
+        s += n;
       }
       if (o.dst < 0) last_work = p;
       if ((rc = timer.mark(1))) return rc;

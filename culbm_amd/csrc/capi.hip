@@ -43,7 +43,9 @@ struct culbm_ctx {
   float scale = 1.0f, inv_scale = 1.0f;
   double state_bound = 0.0;  // feq_bound over the cells of the last culbm_set_state
   int32_t wave_depth = 0;
-  int32_t tiles_x = 0, tiles_y = 0, nchunk = 0;  // mixed policy launch shape
+  // launch shape of k_fused2: tile columns and rows, z planes per workgroup and
+  // z chunks (the mixed policy; the adjoint's forward sweeps on every context)
+  int32_t tiles_x = 0, tiles_y = 0, fzchunk = 0, nchunk = 0;
   // static policy: one instantiated graph per starting state of a full pass
   // (pull: index of the current buffer; in-place: parity of the next step)
   hipGraphExec_t pass_graph[2] = {nullptr, nullptr};
@@ -505,7 +507,7 @@ int fused_pass(culbm_ctx* ctx, int32_t T) {
   fa.a = base_args(ctx);
   fa.tiles_x = ctx->tiles_x;
   fa.tiles_y = ctx->tiles_y;
-  fa.zchunk = ctx->wave_depth;
+  fa.zchunk = ctx->fzchunk;
   fa.nchunk = ctx->nchunk;
   const uint32_t nblk = uint32_t(ctx->tiles_x) * ctx->tiles_y * ctx->nchunk;
   for (int32_t s = 0; s + 1 < T; s += 2) {
@@ -866,9 +868,10 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
   if (blocked(c)) {
     ctx->wave_depth = c.wave_depth > 0 ? std::min(c.wave_depth, c.nz) : auto_depth(ctx);
   }
-  if (c.opt_policy == CULBM_OPT_MIXED) {
-    // z-chunks per tile column: balance the workgroup count over the resident
-    // workgroup slots (LDS-limited) against the two extra A planes a chunk
+  {
+    // k_fused2's launch shape, for the mixed policy and for the adjoint's
+    // forward sweeps: z-chunks per tile column balance the workgroup count over
+    // the resident workgroup slots against the two extra A planes a chunk
     // recomputes
     int ncu = 256;
     hipDeviceProp_t prop;
@@ -887,8 +890,8 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
     ctx->tiles_x = (c.nx + ftx - 1) / ftx;
     ctx->tiles_y = (c.ny + fty - 1) / fty;
     const double tiles = double(ctx->tiles_x) * ctx->tiles_y;
-    if (c.wave_depth > 0) {
-      ctx->wave_depth = std::min(c.wave_depth, c.nz);
+    if (c.opt_policy == CULBM_OPT_MIXED && c.wave_depth > 0) {
+      ctx->fzchunk = std::min(c.wave_depth, c.nz);
     } else {
       double best = 1e30;
       for (int n = 1; n <= std::min(64, c.nz); ++n) {
@@ -898,11 +901,12 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
         const double cost = rounds / (wgs / slots) * (1.0 + 2.0 / len);
         if (cost < best - 1e-9) {
           best = cost;
-          ctx->wave_depth = len;
+          ctx->fzchunk = len;
         }
       }
     }
-    ctx->nchunk = (c.nz + ctx->wave_depth - 1) / ctx->wave_depth;
+    ctx->nchunk = (c.nz + ctx->fzchunk - 1) / ctx->fzchunk;
+    if (c.opt_policy == CULBM_OPT_MIXED) ctx->wave_depth = ctx->fzchunk;  // reported as the depth
   }
 
   // reference defaults: all FLUID + DUMP, rho 1, u 0, rest populations
