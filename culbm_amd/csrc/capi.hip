@@ -564,13 +564,15 @@ int do_steps(culbm_ctx* ctx, int32_t nsteps) {
       ctx->parity ^= 1;
       ctx->step += 1;
     }
+  } else if (c.opt_policy == CULBM_OPT_MIXED) {
+    // two steps per launch across the passes of innerLoop steps (the bits do
+    // not depend on the grouping), one plain step for an odd remainder
+    return fused_pass<S>(ctx, nsteps);
   } else {
     int32_t left = nsteps;
     while (left > 0) {
       const int32_t T = std::min(left, c.inner_loop);
-      const int rc = c.opt_policy == CULBM_OPT_STATIC  ? static_pass<S>(ctx, T)
-                     : c.opt_policy == CULBM_OPT_MIXED ? fused_pass<S>(ctx, T)
-                                                       : wave_pass<S>(ctx, T);
+      const int rc = c.opt_policy == CULBM_OPT_STATIC ? static_pass<S>(ctx, T) : wave_pass<S>(ctx, T);
       if (rc) return rc;
       left -= T;
     }
@@ -1191,6 +1193,8 @@ int culbm_get_info(culbm_ctx* ctx, culbm_info* info) {
   const culbm_config& c = ctx->cfg;
   if (!blocked(c)) {
     info->launches_per_pass = 1;
+  } else if (c.opt_policy == CULBM_OPT_MIXED) {
+    info->launches_per_pass = (c.inner_loop + 1) / 2;  // two steps per launch (pairs run across passes)
   } else {
     int32_t cnt = 0;
     for_each_wave_launch(c.nz, ctx->wave_depth, c.inner_loop, [&](int32_t, int32_t, int32_t) {
