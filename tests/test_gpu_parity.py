@@ -364,6 +364,34 @@ def test_mixed_with_inactive_cells_and_midrun_flag_change():
         assert_bitexact(a, b, "macros")
 
 
+@pytest.mark.parametrize("storage,rounding", [("fp32", "nearest"), ("fp16", "stochastic")])
+def test_mixed_pairs_do_not_depend_on_call_boundaries(storage, rounding):
+    """The mixed policy takes its steps in pairs across passes and calls (one
+    plain step for an odd count): any split of 19 steps into calls, windows
+    included, gives the bits of 19 plain pull steps."""
+    st = obstacle_case(nx=45, ny=26, nz=23)
+    ref = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding)
+    ref.step(19)
+    want = ref.get_ddf()
+    ref.close()
+    for split in ([19], [1, 18], [5, 3, 9, 2], [2, 2, 2, 13], [7, 7, 5]):
+        lat = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding, opt_policy=2, inner_loop=5)
+        for n in split:
+            lat.step(n)
+        assert lat.info()["step"] == 19
+        assert_bitexact(lat.get_ddf(), want, f"{storage} {rounding} split {split}")
+        lat.close()
+    # windows advance by innerLoop: three windows of 7 steps run 10 each
+    lat = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding, opt_policy=2, inner_loop=5)
+    done = [lat.run_window(7)[0] for _ in range(3)]
+    assert done == [10, 10, 10]
+    ref = make(st, init="feq", inv_tau=0.7, storage=storage, rounding=rounding)
+    ref.step(30)
+    assert_bitexact(lat.get_ddf(), ref.get_ddf(), f"{storage} {rounding} windows")
+    lat.close()
+    ref.close()
+
+
 # ---- ragged and tiny domains -------------------------------------------------
 
 RAGGED = [(1, 1, 1), (1, 1, 6), (2, 2, 2), (3, 1, 4), (64, 1, 1), (65, 3, 2), (7, 5, 3), (129, 2, 3),
