@@ -123,8 +123,9 @@ double feq_bound(float rho, float ux, float uy, float uz) {
   return (8.0 / 27.0) * (std::fabs(r - 1.0) + std::fabs(r) * (3.0 * s + 4.5 * s * s + 1.5 * u2));
 }
 
+// FP16 with storage_scale 0: the scale follows the run (BF16's is 1 either way)
 bool auto_scale(const culbm_ctx* ctx) {
-  return ctx->cfg.storage != CULBM_STORAGE_FP32 && ctx->cfg.storage_scale == 0.0f;
+  return ctx->cfg.storage == CULBM_STORAGE_FP16 && ctx->cfg.storage_scale == 0.0f;
 }
 
 void set_scale(culbm_ctx* ctx, float s) {
