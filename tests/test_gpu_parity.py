@@ -247,6 +247,38 @@ def test_set_get_ddf_roundtrip(sp, op, storage):
     assert_bitexact(lat.get_ddf(), f, "roundtrip")
 
 
+@pytest.mark.slow
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+def test_fp16_scale_keeps_headroom_over_a_long_run(rounding):
+    """The benchmark channel (288x272x280, 60 spheres, u_in 0.05) in FP16 with the
+    per-run scale, 6000 steps in place + wavefront from rest: every population
+    stays finite and within a quarter of the scale's range (|f - w| * s below
+    16376 of 65504), and the velocity field tracks FP32's."""
+    st = cases.config_obstacles()
+    steps = 6000
+    lat = make(st, storage="fp16", rounding=rounding, stream_policy=1, opt_policy=3, inner_loop=5)
+    scale = lat.info()["storage_scale"]
+    assert scale == 2.0 ** 16
+    lat.step(steps)
+    f = lat.get_ddf()
+    assert np.isfinite(f).all()
+    assert np.abs(f).max() * scale < 65504.0 / 4
+    lat.dump()
+    u16 = lat.macros()[1]
+    lat.close()
+    ref = make(st, stream_policy=1, opt_policy=3, inner_loop=5)
+    ref.step(steps)
+    ref.dump()
+    u32 = ref.macros()[1]
+    ref.close()
+    fluid = (st.flag & cases.DUMP) != 0
+    assert np.abs(u32[fluid]).max() > 0.04  # the flow has developed
+    err = np.abs(u16[fluid].astype(np.float64) - u32[fluid]).max() / 0.05
+    print(f"fp16 {rounding} {steps} steps: max |f - w| * s = {np.abs(f).max() * scale:.1f} of 65504, "
+          f"max |u16 - u32| / u_in = {err:.2e}", flush=True)
+    assert err < 2e-2, err
+
+
 def test_16bit_scale_is_derived_per_run():
     """FP16's scale is the largest power of two with 16 B s <= 65504, B the
     bound on |f_i - w_i| of the run's state (at least 1/32) or of the
