@@ -50,6 +50,7 @@ FORMS = {
     "fp16_sr_inplace_blocked": (1, 3, "fp16", "FP16-shifted in-place + dynamic blocking, stochastic rounding",
                                 "stochastic"),
     "fp16_sr_mixed": (0, 2, "fp16", "FP16-shifted pull + mixed blocking, stochastic rounding", "stochastic"),
+    "bf16_mixed": (0, 2, "bf16", "BF16-shifted pull + mixed blocking (two steps per launch, LDS)"),
 }
 HEADLINE = "fp32_inplace_blocked"
 
@@ -333,6 +334,18 @@ def cpu_baseline(state, budget_s=12.0):
     }
 
 
+def fastest_forms(head_name, head, forms):
+    """The fastest form of this run per storage format (window medians), the
+    headline included: {storage: {"form": name, "mlups": m, "x_headline": m / headline}}."""
+    rows = [(head_name, head["mlups"])] + [(k, v["mlups"]) for k, v in forms.items()]
+    best = {}
+    for name, m in rows:
+        storage = FORMS[name][2]
+        if storage not in best or m > best[storage][1]:
+            best[storage] = (name, m)
+    return {k: {"form": n, "mlups": round(m, 1), "x_headline": round(m / head["mlups"], 3)} for k, (n, m) in best.items()}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +470,7 @@ def main(argv=None) -> int:
         "adjoint_forward_ratio": adj,
         "precision": prec,
         "forms": forms,
+        "fastest_forms": fastest_forms(ns.form, head, forms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -29,3 +29,13 @@ def test_more_than_one_gpu_is_refused_with_a_message():
                        timeout=120, cwd=ROOT)
     assert p.returncode != 0
     assert "one GPU" in (p.stdout + p.stderr) or "1 GPU" in (p.stdout + p.stderr)
+
+
+def test_fastest_forms_picks_per_storage():
+    head = {"mlups": 100.0}
+    forms = {"fp32_mixed": {"mlups": 140.0}, "fp32_plain_pull": {"mlups": 90.0}, "fp16_mixed": {"mlups": 210.0},
+             "fp16_plain_pull": {"mlups": 190.0}, "bf16_inplace_blocked": {"mlups": 170.0}}
+    best = bench.fastest_forms("fp32_inplace_blocked", head, forms)
+    assert best["fp32"] == {"form": "fp32_mixed", "mlups": 140.0, "x_headline": 1.4}
+    assert best["fp16"]["form"] == "fp16_mixed" and best["bf16"]["form"] == "bf16_inplace_blocked"
+    assert bench.fastest_forms("fp32_inplace_blocked", head, {})["fp32"]["form"] == "fp32_inplace_blocked"
