@@ -376,6 +376,15 @@ int prof_end(culbm_ctx* ctx) {
 template <class S, bool SR, bool DUMP>
 void launch_step_kernel(const culbm_ctx* ctx, const StepArgs& a, int parity) {
   if (ctx->inplace) {
+    // even nx: x-face consumers read in place (no x ghost copies); rows then
+    // start at even lanes, which keeps x = 0 / 1 and nx-2 / nx-1 in one wave
+    if (ctx->cfg.nx % 2 == 0) {
+      if (parity)
+        hipLaunchKernelGGL((k_twist<S, 1, SR, DUMP, true>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+      else
+        hipLaunchKernelGGL((k_twist<S, 0, SR, DUMP, true>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
+      return;
+    }
     if (parity)
       hipLaunchKernelGGL((k_twist<S, 1, SR, DUMP>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a);
     else

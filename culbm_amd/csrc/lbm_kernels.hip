@@ -304,18 +304,31 @@ __device__ __forceinline__ TwistRows twist_rows(const StepArgs& a, const Cell& k
 }
 
 // Loads of a step of parity P (1 = O: slot i, 0 = E: slot 26-i) at x + p(-c_i).
-template <class S, int P>
+//
+// XR (x faces read in place): a consumer clamped along x (c_x = +1 at x = 0,
+// c_x = -1 at x = nx-1) pulls its own value of the step before, which that
+// step stored regularly at x + p(c_x), one cell inward: it is read there
+// instead of from a ghost copy.  That location belongs to the neighbouring
+// cell's load/store set, so it is overwritten in the same step; the read comes
+// first because the two cells are adjacent lanes of one wave (x = 0 and 1,
+// nx-2 and nx-1: rows start at even lanes when nx is even) and every load of
+// a wave returns before any store is issued (the update waits for all 27).
+template <class S, int P, bool XR = false>
 struct TwistLoad {
   const char* base;
   int64_t dsb;
   TwistRows r;
   float inv_s, nz;
+  uint32_t xp, xm;  // XR: byte x offsets of the c_x = +1 / -1 loads (B or 0 at the faces)
   template <int I>
   __device__ __forceinline__ void run(float* f) const {
     constexpr int cx = cx_of(I), cy = cy_of(I), cz = cz_of(I);
     constexpr int slot = P == 1 ? I : 26 - I;
     constexpr uint32_t ox = cx == -1 ? sizeof(typename S::T) : 0;
-    f[I] = S::dec_fast(ld_at<typename S::T>(base + slot * dsb, r.o[cy == -1][cz == -1] + ox), inv_s, nz);
+    uint32_t off = r.o[cy == -1][cz == -1] + ox;
+    if constexpr (XR && cx == 1) off = r.o[cy == -1][cz == -1] + xp;
+    if constexpr (XR && cx == -1) off = r.o[cy == -1][cz == -1] + xm;
+    f[I] = S::dec_fast(ld_at<typename S::T>(base + slot * dsb, off), inv_s, nz);
     if constexpr (I < 26) run<I + 1>(f);
   }
 };
@@ -329,7 +342,7 @@ struct TwistLoad {
 // every row) are stored next to each regular store; the rest (y or z faces:
 // whole rows and planes) by TwistGhostYZ afterwards, from the values just
 // stored, so that no population has to stay in a register for them.
-template <class S, int P, bool SR>
+template <class S, int P, bool SR, bool XG = true>
 struct TwistStore {
   char* base;
   int64_t dsb;
@@ -344,7 +357,7 @@ struct TwistStore {
     constexpr uint32_t ox = cx == 1 ? sizeof(T) : 0;
     const T v = e.template get<I>(f[I]);
     st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, v);
-    if constexpr (cx != 0) {
+    if constexpr (XG && cx != 0) {
       constexpr uint32_t qx = cx == -1 ? sizeof(T) : 0;
       if (cx == 1 ? (fx & 1u) : (fx >> 1)) st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + qx, v);
     }
@@ -355,7 +368,7 @@ struct TwistStore {
 // Ghost copies that involve a y or z face (clamped along y and/or z, with or
 // without x).  The values are read back from the regular slots this cell has
 // just written (its own stores, so the same bits), all loads first.
-template <class S, int P>
+template <class S, int P, bool XG = true>
 struct TwistGhostYZ {
   using T = typename S::T;
   char* base;
@@ -384,7 +397,7 @@ struct TwistGhostYZ {
       const int32_t sx = cx == 1 ? B : 0, sy = cy == 1 ? r.row : 0, sz = cz == 1 ? r.plane : 0;
       const int32_t qx = cx == -1 ? B : 0, qy = cy == -1 ? r.row : 0, qz = cz == -1 ? r.plane : 0;
       for (uint32_t m = 2; m < 8; ++m)
-        if ((m & ~gm) == 0)
+        if ((m & ~gm) == 0 && (XG || !(m & 1u)))  // without x ghosts, x-clamped consumers read in place
           st_at<T>(p, r.lb + uint32_t(((m & 1u) ? qx : sx) + ((m & 2u) ? qy : sy) + ((m & 4u) ? qz : sz)),
                    v[I]);
     }
@@ -392,10 +405,13 @@ struct TwistGhostYZ {
   }
 };
 
-template <class S, int P>
+template <class S, int P, bool XR = false>
 __device__ __forceinline__ void twist_load_all(const StepArgs& a, const Cell& k, const void* buf,
                                                float* f) {
-  TwistLoad<S, P> t;
+  TwistLoad<S, P, XR> t;
+  constexpr uint32_t B = sizeof(typename S::T);
+  t.xp = k.x == 0 ? B : 0u;
+  t.xm = k.x == a.nx - 1 ? 0u : B;
   t.base = static_cast<const char*>(buf);
   t.dsb = a.dstride * int64_t(sizeof(typename S::T));
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
@@ -404,10 +420,10 @@ __device__ __forceinline__ void twist_load_all(const StepArgs& a, const Cell& k,
   t.template run<0>(f);
 }
 
-template <class S, int P, bool SR = false>
+template <class S, int P, bool SR = false, bool XG = true>
 __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k, void* buf,
                                                 const float* f, const Encoder<S, SR>& e) {
-  TwistStore<S, P, SR> t;
+  TwistStore<S, P, SR, XG> t;
   t.base = static_cast<char*>(buf);
   t.dsb = a.dstride * int64_t(sizeof(typename S::T));
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
@@ -417,7 +433,7 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   const uint32_t fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
   const uint32_t fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
   if (fy | fz) {  // waves on a y or z face only
-    TwistGhostYZ<S, P> g;
+    TwistGhostYZ<S, P, XG> g;
     g.base = t.base;
     g.dsb = t.dsb;
     g.r = t.r;
@@ -435,19 +451,19 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   }
 }
 
-template <class S, int P, bool SR, bool DUMP = false>
+template <class S, int P, bool SR, bool DUMP = false, bool XR = false>
 __global__ __launch_bounds__(kBlock) void k_twist(const StepArgs a) {
   const Cell k = locate(a, cell_block(a));
   if (!k.ok) return;
   // twisted loads stay inside the padded layout for every cell: issue them
   // before the flag arrives
   float f[27];
-  twist_load_all<S, P>(a, k, a.dst, f);
+  twist_load_all<S, P, XR>(a, k, a.dst, f);
   EquIn q = equ_prefetch(a, k.c, k.x);
   if constexpr (DUMP) fused_dump(a, k.c, f, q);
   const uint32_t fl = a.sflag[k.c];
   update_cell(fl, a, k.c, q, f);
-  if (fl & kStore) twist_store_all<S, P, SR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
+  if (fl & kStore) twist_store_all<S, P, SR, !XR>(a, k, a.dst, f, make_encoder<S, SR>(a, k.c, a.state));
 }
 
 // ---------------------------------------------------------------------------
@@ -714,8 +730,9 @@ __global__ __launch_bounds__(kBlock) void k_dump(const DumpArgs d) {
   if (!k.ok) return;
   float f[27];
   if constexpr (MODE == 0) make_gather<S>(d.a, k, d.a.src).template run<0>(f);
-  if constexpr (MODE == 1) twist_load_all<S, 1>(d.a, k, d.a.src, f);
-  if constexpr (MODE == 2) twist_load_all<S, 0>(d.a, k, d.a.src, f);
+  // between steps the regular locations hold every value (ghost copies may be absent)
+  if constexpr (MODE == 1) twist_load_all<S, 1, true>(d.a, k, d.a.src, f);
+  if constexpr (MODE == 2) twist_load_all<S, 0, true>(d.a, k, d.a.src, f);
   const Moments m = moments(f);
   const uint32_t fl = d.flag[k.c];
   if (fl & kDumpRho) d.rho[k.c] = m.rho;

@@ -11,7 +11,12 @@ plane of the same launch still reads.
 Model.  A launch updates every cell of its z-planes, so hazards between
 launches live along z; within a launch the twisted layout's per-cell slot sets
 are disjoint (each slot is read and written by one cell, which the streaming
-index-tag tests pin on the GPU).  Along z only c_z of a direction matters, so
+index-tag tests pin on the GPU).  One exception stays inside a wave: for even
+nx the cells on an x face read their clamped x value at the neighbour one cell
+inward (no x ghost copies), a slot that neighbour then overwrites; the two are
+adjacent lanes of one wave, whose loads all return before any of its stores
+issue, so no ordering between launches or workgroups is involved.  Along z only
+c_z of a direction matters, so
 the 27 slots of a cell fall into three classes c = c_z in {-1, 0, +1}:
 
   in place, O step (parity 1): load class c at plane z + p(-c), store class -c
