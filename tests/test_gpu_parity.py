@@ -144,8 +144,13 @@ POLICIES = [(0, 0), (1, 0), (0, 3), (1, 3), (0, 1), (1, 1), (0, 2)]
 
 
 @pytest.mark.parametrize("sp,op", POLICIES)
-def test_stream_index_tags(sp, op):
-    nx, ny, nz = 13, 9, 11
+@pytest.mark.parametrize("nx", [13, 14])
+def test_stream_index_tags(sp, op, nx):
+    """Every population carries its own index; after 1 and 4 steps each cell
+    holds exactly the indices the clamped pull brings it.  Odd nx runs the
+    in-place step with x ghost stores, even nx the one whose x-face cells read
+    their clamped value in place."""
+    ny, nz = 9, 11
     st = cases.channel(nx, ny, nz)
     st.flag[...] = cases.LOAD | cases.STORE
     lat = Lattice(LatticeConfig(domain=(nx, ny, nz), stream_policy=sp, opt_policy=op, inner_loop=3,
@@ -427,7 +432,7 @@ def test_mixed_pairs_do_not_depend_on_call_boundaries(storage, rounding):
 # ---- ragged and tiny domains -------------------------------------------------
 
 RAGGED = [(1, 1, 1), (1, 1, 6), (2, 2, 2), (3, 1, 4), (64, 1, 1), (65, 3, 2), (7, 5, 3), (129, 2, 3),
-          (33, 9, 17)]
+          (33, 9, 17), (34, 9, 17), (66, 3, 2), (130, 2, 3)]
 
 
 @pytest.mark.parametrize("dom", RAGGED)
@@ -521,11 +526,13 @@ def test_16bit_ragged_domains_match_the_oracle(dom):
 
 
 @pytest.mark.parametrize("sp,op", POLICIES)
-def test_16bit_stream_index_tags(sp, op):
+@pytest.mark.parametrize("nx", [5, 6])
+def test_16bit_stream_index_tags(sp, op, nx):
     """Cell-id tags through pure streaming in FP16 with scale 1 (every tag an
-    exact binary16 integer): the 16-bit addressing, face clamp and ghost copies
-    against the clamped-streaming expectation."""
-    nx, ny, nz = 6, 3, 4
+    exact binary16 integer): the 16-bit addressing, face clamp, ghost copies
+    (odd nx) and in-place x-face reads (even nx) against the clamped-streaming
+    expectation."""
+    ny, nz = 3, 4
     st = cases.channel(nx, ny, nz)
     st.flag[...] = cases.LOAD | cases.STORE
     lat = Lattice(LatticeConfig(domain=(nx, ny, nz), stream_policy=sp, opt_policy=op, inner_loop=3,
