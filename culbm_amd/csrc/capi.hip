@@ -26,6 +26,7 @@ struct culbm_ctx {
 
   int64_t n = 0;           // cells
   int32_t px = 0, py = 0, pz = 0;
+  int64_t gcol = 0;        // in-place, rows packed: first ghost-column element per direction
   int64_t dstride = 0;     // population elements per direction
   int32_t bpp = 4;         // bytes per population
   bool inplace = false;
@@ -144,8 +145,13 @@ void set_scale(culbm_ctx* ctx, float s) {
 // Population layout of a configuration: pitches and direction stride.  The one
 // place that defines it, so the 32-bit offset check of culbm_validate tests
 // exactly what culbm_create allocates.
+int32_t bytes_per_pop(int32_t storage) {
+  return (storage == CULBM_STORAGE_FP16 || storage == CULBM_STORAGE_BF16) ? 2 : 4;
+}
+
 struct Layout {
   int32_t px, py, pz;
+  int64_t gcol;     // packed rows: first element of the ghost column region (0: padded rows)
   int64_t dstride;  // elements per direction array
 };
 
@@ -166,17 +172,25 @@ Layout layout_of(const culbm_config& c) {
     L.px = int32_t(round_up(int64_t(c.nx) + 1, align));
     L.py = c.ny + 1;
     L.pz = c.nz + 1;
+    // FP32 with even nx: rows packed at pitch nx, the ghost column (the c_x = +1
+    // stores of x = nx-1) in a (ny+1) x (nz+1) region after the planes.  Even nx
+    // reads x faces in place, so no load ever reaches x = nx.  Measured against
+    // the padded pitch, FP32 in place + wavefront, two runs per setting in
+    // alternating order: +2-4 % at 288x272x280, +14-19 % at 128^3, +5 % at
+    // 192^3, +4 % at 384^3, +1 % at 512^3, +-0.5 % at 256^3; unblocked +6.5 % at
+    // 288x272x280.  16-bit storage, whose 576-byte rows then share lines, lost
+    // 4-6 % and keeps padded rows.
+    if (bytes_per_pop(c.storage) == 4 && c.nx % 2 == 0) {
+      L.px = c.nx;
+      L.gcol = int64_t(L.px) * L.py * L.pz;
+    }
   } else {
     L.px = c.nx;
     L.py = c.ny;
     L.pz = c.nz;
   }
-  L.dstride = round_up(int64_t(L.px) * L.py * L.pz, 256);
+  L.dstride = round_up(int64_t(L.px) * L.py * L.pz + (L.gcol ? int64_t(L.py) * L.pz : 0), 256);
   return L;
-}
-
-int32_t bytes_per_pop(int32_t storage) {
-  return (storage == CULBM_STORAGE_FP16 || storage == CULBM_STORAGE_BF16) ? 2 : 4;
 }
 
 }  // namespace
@@ -225,7 +239,7 @@ int culbm_validate(const culbm_config* c, char* msg, size_t msg_len) {
     // the kernels address one direction array with 32-bit per-lane byte offsets
     // (cell index x + px * (y + py * z) of the layout culbm_create allocates)
     const Layout L = layout_of(*c);
-    const int64_t cells = int64_t(L.px) * L.py * L.pz;
+    const int64_t cells = int64_t(L.px) * L.py * L.pz + (L.gcol ? int64_t(L.py) * L.pz : 0);
     const int64_t per_dir = cells * bytes_per_pop(c->storage);
     if (cells > (int64_t(1) << 32) - 1 || per_dir > (int64_t(1) << 32))  // the last element starts at per_dir - bpp
       return bad(CULBM_ERR_INVALID,
@@ -305,6 +319,7 @@ StepArgs base_args(const culbm_ctx* ctx) {
   a.nz = ctx->cfg.nz;
   a.px = ctx->px;
   a.py = ctx->py;
+  a.gcol = uint32_t(ctx->gcol);
   a.z0 = 0;
   a.z1 = ctx->cfg.nz;
   a.ncells = uint32_t(ctx->n);
@@ -609,9 +624,14 @@ template <class S>
 int upload_plain(culbm_ctx* ctx, const float* dplain) {
   StepArgs a = full_args(ctx);
   if (ctx->inplace) {
-    // written with the E-step store rules, so the next step is an O step
-    hipLaunchKernelGGL((k_plain_to_twist<S, 0>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
-                       dplain, ctx->buf[0]);
+    // written with the E-step store rules, so the next step is an O step; the x
+    // ghost copies only where the steps read them (odd nx)
+    if (ctx->cfg.nx % 2 == 0)
+      hipLaunchKernelGGL((k_plain_to_twist<S, 0, false>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                         dplain, ctx->buf[0]);
+    else
+      hipLaunchKernelGGL((k_plain_to_twist<S, 0, true>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a,
+                         dplain, ctx->buf[0]);
     ctx->parity = 1;
   } else {
     for (int b = 0; b < 2; ++b)
@@ -864,6 +884,7 @@ int culbm_create(const culbm_config* cfg, culbm_ctx** out) {
     ctx->px = L.px;
     ctx->py = L.py;
     ctx->pz = L.pz;
+    ctx->gcol = L.gcol;
     ctx->dstride = L.dstride;
   }
 

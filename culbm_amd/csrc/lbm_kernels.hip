@@ -338,6 +338,13 @@ struct TwistLoad {
 
 // Stores of a step of parity P (1 = O: slot 26-i, 0 = E: slot i) at x + p(c_i).
 //
+// The c_x = +1 stores of the last cell of a row land at x = nx, the ghost
+// column.  With padded rows it is column nx of the row; with rows packed at
+// pitch nx (a.gcol != 0) it is a separate (ny+1) x (nz+1) region per direction
+// array, index a.gcol + y' + py z', so that no store reaches the next row.  Only
+// the layout conversion and the moment sums read it back (its values are the
+// last cell's post-collision c_x = +1 populations); no step does.
+//
 // Ghost copies of a face cell: a consumer is clamped onto this cell on axis a
 // when x_a - c_a leaves the domain (c_a = +1 on the low face, -1 on the high
 // face); its load slot on that axis is at p(-c_a) instead of p(c_a).  The
@@ -352,6 +359,9 @@ struct TwistStore {
   TwistRows r;
   Encoder<S, SR> e;
   uint32_t fx;  // bit 0: x = 0; bit 1: x = nx-1
+  bool gl;      // this lane's c_x = +1 stores go to the packed layout's ghost column
+  uint32_t gb;  // byte offset of its ghost-column entry (y, z)
+  uint32_t gpy; // byte offset of one ghost-column plane
   template <int I>
   __device__ __forceinline__ void run(const float* f) const {
     using T = typename S::T;
@@ -359,7 +369,10 @@ struct TwistStore {
     constexpr int slot = P == 1 ? 26 - I : I;
     constexpr uint32_t ox = cx == 1 ? sizeof(T) : 0;
     const T v = e.template get<I>(f[I]);
-    st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + ox, v);
+    uint32_t off = r.o[cy == 1][cz == 1] + ox;
+    if constexpr (cx == 1)
+      if (gl) off = gb + (cy == 1 ? uint32_t(sizeof(T)) : 0u) + (cz == 1 ? gpy : 0u);
+    st_at<T>(base + slot * dsb, off, v);
     if constexpr (XG && cx != 0) {
       constexpr uint32_t qx = cx == -1 ? sizeof(T) : 0;
       if (cx == 1 ? (fx & 1u) : (fx >> 1)) st_at<T>(base + slot * dsb, r.o[cy == 1][cz == 1] + qx, v);
@@ -370,7 +383,10 @@ struct TwistStore {
 
 // Ghost copies that involve a y or z face (clamped along y and/or z, with or
 // without x).  The values are read back from the regular slots this cell has
-// just written (its own stores, so the same bits), all loads first.
+// just written (its own stores, so the same bits), all loads first.  A c_x = +1
+// value of the last cell of a row has no such consumer unless it is clamped
+// along x as well (nx = 1): the others would sit at x = nx, outside the domain,
+// so they are not written (its read-back value, from x = nx, goes unused).
 template <class S, int P, bool XG = true>
 struct TwistGhostYZ {
   using T = typename S::T;
@@ -399,8 +415,9 @@ struct TwistGhostYZ {
       char* p = base + slot * dsb;
       const int32_t sx = cx == 1 ? B : 0, sy = cy == 1 ? r.row : 0, sz = cz == 1 ? r.plane : 0;
       const int32_t qx = cx == -1 ? B : 0, qy = cy == -1 ? r.row : 0, qz = cz == -1 ? r.plane : 0;
+      const bool out_x = cx == 1 && (fx >> 1);  // consumer x + 1 = nx unless clamped along x
       for (uint32_t m = 2; m < 8; ++m)
-        if ((m & ~gm) == 0 && (XG || !(m & 1u)))  // without x ghosts, x-clamped consumers read in place
+        if ((m & ~gm) == 0 && (XG || !(m & 1u)) && !(out_x && !(m & 1u)))
           st_at<T>(p, r.lb + uint32_t(((m & 1u) ? qx : sx) + ((m & 2u) ? qy : sy) + ((m & 4u) ? qz : sz)),
                    v[I]);
     }
@@ -432,6 +449,10 @@ __device__ __forceinline__ void twist_store_all(const StepArgs& a, const Cell& k
   t.r = twist_rows<sizeof(typename S::T)>(a, k);
   t.e = e;
   t.fx = uint32_t(k.x == 0) | (uint32_t(k.x == a.nx - 1) << 1);
+  constexpr uint32_t B = sizeof(typename S::T);
+  t.gl = a.gcol != 0 && k.x == a.nx - 1;
+  t.gb = (a.gcol + uint32_t(k.y) + uint32_t(a.py) * uint32_t(k.z)) * B;
+  t.gpy = uint32_t(a.py) * B;
   t.template run<0>(f);
   const uint32_t fy = uint32_t(k.y == 0) | (uint32_t(k.y == a.ny - 1) << 1);
   const uint32_t fz = uint32_t(k.z == 0) | (uint32_t(k.z == a.nz - 1) << 1);
@@ -772,8 +793,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_to_plain(const StepArgs a, cons
   for (int i = 0; i < 27; ++i) plain[i * n + k.c] = S::dec(b[i * a.dstride + k.l], a.inv_scale);
 }
 
-// plain FP32 -> in-place layout as left by a step of parity P (store rules of P)
-template <class S, int P>
+// plain FP32 -> in-place layout as left by a step of parity P (store rules of P);
+// XG: with the x ghost copies (odd nx; for even nx the steps read x faces in place)
+template <class S, int P, bool XG>
 __global__ __launch_bounds__(kBlock) void k_plain_to_twist(const StepArgs a, const float* plain,
                                                            void* out) {
   const Cell k = locate(a, cell_block(a));
@@ -782,7 +804,20 @@ __global__ __launch_bounds__(kBlock) void k_plain_to_twist(const StepArgs a, con
   float f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) f[i] = plain[i * n + k.c];
-  twist_store_all<S, P>(a, k, out, f, make_encoder<S, false>(a, 0u, 0u));
+  twist_store_all<S, P, false, XG>(a, k, out, f, make_encoder<S, false>(a, 0u, 0u));
+}
+
+// Element index of population i of cell k in the in-place layout left by a step
+// of parity P (its regular store slot; the ghost column for a c_x = +1 value of
+// the last cell of a row when the rows are packed).
+template <int P>
+__device__ __forceinline__ int64_t twist_stored_index(const StepArgs& a, const Cell& k, int i) {
+  const int slot = P == 1 ? 26 - i : i;
+  const uint32_t oy = cy_of(i) == 1 ? 1u : 0u, oz = cz_of(i) == 1 ? 1u : 0u;
+  if (cx_of(i) == 1 && a.gcol != 0 && k.x == a.nx - 1)
+    return slot * a.dstride + int64_t(a.gcol + (uint32_t(k.y) + oy) + uint32_t(a.py) * (uint32_t(k.z) + oz));
+  const uint32_t o = (cx_of(i) == 1 ? 1u : 0u) + oy * uint32_t(a.px) + oz * uint32_t(a.px) * uint32_t(a.py);
+  return slot * a.dstride + int64_t(k.l + o);
 }
 
 // in-place layout left by a step of parity P -> plain FP32 (regular slots)
@@ -793,13 +828,8 @@ __global__ __launch_bounds__(kBlock) void k_twist_to_plain(const StepArgs a, con
   if (!k.ok) return;
   const int64_t n = int64_t(a.nx) * a.ny * a.nz;
   const typename S::T* b = static_cast<const typename S::T*>(in);
-  const int32_t sy = a.px, sz = a.px * a.py;
 #pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    const int slot = P == 1 ? 26 - i : i;
-    const int32_t o = (cx_of(i) == 1 ? 1 : 0) + (cy_of(i) == 1 ? sy : 0) + (cz_of(i) == 1 ? sz : 0);
-    plain[i * n + k.c] = S::dec(b[slot * a.dstride + (k.l + uint32_t(o))], a.inv_scale);
-  }
+  for (int i = 0; i < 27; ++i) plain[i * n + k.c] = S::dec(b[twist_stored_index<P>(a, k, i)], a.inv_scale);
 }
 
 // Stored populations of cell k (the post-collision view of culbm_get_ddf):
@@ -807,16 +837,13 @@ __global__ __launch_bounds__(kBlock) void k_twist_to_plain(const StepArgs a, con
 template <class S, int MODE>
 __device__ __forceinline__ void stored_cell(const StepArgs& a, const Cell& k, const void* buf, float* f) {
   const typename S::T* b = static_cast<const typename S::T*>(buf);
-  const int32_t sy = a.px, sz = a.px * a.py;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     if constexpr (MODE == 0) {
       f[i] = S::dec(b[i * a.dstride + k.l], a.inv_scale);
     } else {
       constexpr int P = MODE == 1 ? 1 : 0;
-      const int slot = P == 1 ? 26 - i : i;
-      const int32_t o = (cx_of(i) == 1 ? 1 : 0) + (cy_of(i) == 1 ? sy : 0) + (cz_of(i) == 1 ? sz : 0);
-      f[i] = S::dec(b[slot * a.dstride + (k.l + uint32_t(o))], a.inv_scale);
+      f[i] = S::dec(b[twist_stored_index<P>(a, k, i)], a.inv_scale);
     }
   }
 }

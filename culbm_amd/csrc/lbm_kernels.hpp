@@ -129,6 +129,9 @@ constexpr uint32_t kOrderRuns = 2;      // runs of consecutive workgroups per XC
 struct StepArgs {
   int32_t nx, ny, nz;
   int32_t px, py;        // layout pitches (cells per row, rows per plane)
+  uint32_t gcol;         // in-place rows packed at pitch nx: first element of the ghost column
+                         // (the c_x = +1 stores of x = nx-1) in each direction array; 0: rows
+                         // padded, the ghost column is x = nx
   int32_t z0, z1;        // plane range of this launch
   uint32_t ncells;       // (z1-z0)*nx*ny
   int64_t dstride;       // elements between consecutive directions
