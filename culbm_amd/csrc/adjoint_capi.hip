@@ -175,21 +175,36 @@ struct AdjBuffers {
   }
 };
 
-// Event pairs per phase (0 sweep, 1 recompute, 2 reverse), summed at the end.
+// Event pairs per phase (0 sweep, 1 recompute, 2 reverse), summed at the end:
+// one pair per run of consecutive operations of the same phase (a pair around
+// every launch would add its own gaps between the launches it measures).
 struct AdjTimer {
   culbm_ctx* ctx;
   bool on;
   std::vector<std::pair<hipEvent_t, int>> ev;
+  int open = -1;  // phase of the pair begun last and not yet ended
   ~AdjTimer() {
     for (auto& e : ev) (void)hipEventDestroy(e.first);
   }
-  int mark(int phase) {
-    if (!on) return CULBM_OK;
+  int record(int phase) {
     hipEvent_t e;
     HIPCHK(ctx, hipEventCreate(&e));
     ev.push_back({e, phase});
     HIPCHK(ctx, hipEventRecord(e, ctx->stream));
     return CULBM_OK;
+  }
+  int enter(int phase) {  // the work enqueued next belongs to `phase`
+    if (!on || open == phase) return CULBM_OK;
+    if (open >= 0)
+      if (const int rc = record(open)) return rc;
+    open = phase;
+    return record(phase);
+  }
+  int close() {  // ends the running pair
+    if (!on || open < 0) return CULBM_OK;
+    const int p = open;
+    open = -1;
+    return record(p);
   }
   int sum(float* ms) {  // pairs (begin, end) in order of recording
     for (int k = 0; k < 3; ++k) ms[k] = 0.0f;
@@ -389,7 +404,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
 
   AdjTimer timer{ctx, sp.timing != 0, {}};
   // ---- first sweep: J, the sweep checkpoints, f^N
-  if ((rc = timer.mark(0))) return rc;
+  if ((rc = timer.enter(0))) return rc;
   const void* cur = slot[0];
   size_t next_store = 0;
   auto stored_at = [&](int32_t t) { return next_store < plan.sweep.size() && plan.sweep[next_store].first == t; };
@@ -406,7 +421,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
     cur = dst;
     t += n;
   }
-  if ((rc = timer.mark(0))) return rc;
+  if ((rc = timer.close())) return rc;
   HIPCHK(ctx, hipGetLastError());
   // the context moves on to f^N
   hipLaunchKernelGGL((k_pull_to_plain<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, a, cur, mu[1]);
@@ -432,7 +447,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   for (const AdjOp& o : plan.ops) {
     const void* src = o.src >= 0 ? slot[size_t(o.src)] : last_work;
     if (o.kind == 0) {
-      if ((rc = timer.mark(1))) return rc;
+      if ((rc = timer.enter(1))) return rc;
       const void* p = src;
       for (int32_t s = 0; s < o.steps;) {
         const int32_t n = s + 2 <= o.steps ? 2 : 1;  // pairs of steps in one launch
@@ -445,9 +460,8 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
         s += n;
       }
       if (o.dst < 0) last_work = p;
-      if ((rc = timer.mark(1))) return rc;
     } else {
-      if ((rc = timer.mark(2))) return rc;
+      if ((rc = timer.enter(2))) return rc;
       j.a.src = src;
       j.mu_in = mu[mc];
       j.mu_out = mu[mc ^ 1];
@@ -455,9 +469,9 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
       hipLaunchKernelGGL((k_adjoint<S>), dim3(a.nblk), dim3(kBlock), 0, ctx->stream, j);
       ++launches;
       mc ^= 1;
-      if ((rc = timer.mark(2))) return rc;
     }
   }
+  if ((rc = timer.close())) return rc;
   HIPCHK(ctx, hipGetLastError());
 
   // ---- gradients back to the host
