@@ -239,6 +239,7 @@ int adjoint_impl(culbm_ctx* ctx, const culbm_adjoint_spec& sp, const double* wei
   StepArgs a = base_args(ctx);
   a.px = c.nx;
   a.py = c.ny;
+  a.gcol = 0;
   a.dstride = n;
   set_range(a, 0, c.nz, true, ctx->bpp == 2);
 
