@@ -370,7 +370,7 @@ struct TwistStore {
     constexpr uint32_t ox = cx == 1 ? sizeof(T) : 0;
     const T v = e.template get<I>(f[I]);
     uint32_t off = r.o[cy == 1][cz == 1] + ox;
-    if constexpr (cx == 1)
+    if constexpr (cx == 1 && sizeof(T) == 4)  // only FP32 rows are ever packed
       if (gl) off = gb + (cy == 1 ? uint32_t(sizeof(T)) : 0u) + (cz == 1 ? gpy : 0u);
     st_at<T>(base + slot * dsb, off, v);
     if constexpr (XG && cx != 0) {
