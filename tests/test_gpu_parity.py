@@ -236,8 +236,12 @@ def test_dump_writes_only_flagged_cells():
 
 @pytest.mark.parametrize("sp,op", [(0, 0), (1, 0), (1, 3)])
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
-def test_set_get_ddf_roundtrip(sp, op, storage):
-    st = obstacle_case(nx=19, ny=17, nz=15)
+@pytest.mark.parametrize("nx", [19, 20])
+def test_set_get_ddf_roundtrip(sp, op, storage, nx):
+    """set_ddf then get_ddf returns the populations (16-bit: as the codec
+    rounds them); even nx with FP32 in place goes through the packed rows'
+    ghost-column region for the last cell of every row."""
+    st = obstacle_case(nx=nx, ny=17, nz=15)
     lat = make(st, stream_policy=sp, opt_policy=op, storage=storage)
     rng = np.random.default_rng(3)
     f = rng.normal(0, 1e-2, (27,) + st.shape).astype(np.float32)
