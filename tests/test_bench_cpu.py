@@ -39,3 +39,13 @@ def test_fastest_forms_picks_per_storage():
     assert best["fp32"] == {"form": "fp32_mixed", "mlups": 140.0, "x_headline": 1.4}
     assert best["fp16"]["form"] == "fp16_mixed" and best["bf16"]["form"] == "bf16_inplace_blocked"
     assert bench.fastest_forms("fp32_inplace_blocked", head, {})["fp32"]["form"] == "fp32_inplace_blocked"
+
+
+def test_fp16_claim_forms_are_stochastic_fp16():
+    """The >= 1.7x at <= 1e-3 claim may only take an FP16 form with stochastic
+    rounding (round to nearest misses the error bar); they all store the same
+    bits, which the GPU variant tests pin."""
+    assert bench.FP16_CLAIMS
+    for form in bench.FP16_CLAIMS:
+        spec = bench.FORMS[form]
+        assert spec[2] == "fp16" and len(spec) > 4 and spec[4] == "stochastic", form
